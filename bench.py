@@ -1,0 +1,224 @@
+"""Benchmark: D2Q9 TRT (magic Lambda = 1/4) on a periodic 8192 x 8192 grid, one MI355X.
+
+One step = one fused pull-stream + collide pass over the whole grid (the reference's
+lbm_stream + collide_trt, src/periodic_lbm.f90:32-129 + src/collision_trt.F90:41-160).
+The headline value is FP64 MLUPS (reference arithmetic, bit-exact); the shifted FP32 and
+FP16 storage forms are timed the same way and their velocity / mass error against the
+FP64 run after 5000 steps is stated beside their MLUPS.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--acc-steps S] [--no-cpu]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+N = 8192
+NU, U0, NOISE, SEED = 0.01, 0.05, 1e-3, 7
+MAGIC = 0.25
+BYTES_PER_LU = {"f64": 144, "f64s": 144, "f32s": 72, "f16s": 36}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--acc-steps", type=int, default=5000,
+                    help="steps of the FP64 vs FP32/FP16 accuracy comparison (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--n", type=int, default=N, help="grid size (default 8192)")
+    return ap.parse_args()
+
+
+def initial_fields(P, n):
+    """Taylor-Green (u0=0.05, nu=0.01, k=2pi/n) plus U(+-1e-3) noise of seed 7."""
+    import numpy as np
+    k = 2 * np.pi / n
+    p, ux, uy = P.TaylorGreen(n, n, k, k, U0, NU).eval(0.0)
+    P.add_noise(ux, uy, SEED, NOISE)
+    return np.asfortranarray(p / P.csqr + 1.0), ux, uy
+
+
+def timed_run(P, torch, lat, model, steps, warmup):
+    """Warm up, then time exactly `steps` fused steps with HIP events on the stream the
+    kernels are launched on.  Returns (ms per step, final iold)."""
+    iold, inew = 0, 1
+    iold, inew = lat.run(model, warmup, iold, inew)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    iold, inew = lat.run(model, steps, iold, inew)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, iold
+
+
+def gpu_leg(args):
+    import numpy as np
+    import torch
+    import __graft_entry__ as g
+
+    P = g.load_package()
+    torch.cuda.set_device(0)
+    dev_name = torch.cuda.get_device_name(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    n = args.n
+    cells = n * n
+    rho, ux, uy = initial_fields(P, n)
+    d_rho, d_ux, d_uy = (torch.from_numpy(np.ravel(a, order="F")).cuda() for a in (rho, ux, uy))
+    omega = 1.0 / (3.0 * NU + 0.5)  # set_properties(nu, dt=1): omega = dt/(3 nu + dt/2)
+
+    results = {}
+    for storage in ("f64", "f32s", "f16s"):
+        lat = P.Lattice(n, n, 2, storage)
+        lat.set_stream(stream)
+        lat.set_params(omega, MAGIC)
+        lat.equilibrium_init_dev(0, d_rho.data_ptr(), d_ux.data_ptr(), d_uy.data_ptr())
+        torch.cuda.synchronize()
+        ms, _ = timed_run(P, torch, lat, "trt", args.steps, args.warmup)
+        mlups = cells / (ms * 1e-3) / 1e6
+        results[storage] = {"ms_per_step": ms, "mlups": mlups,
+                            "gbs": mlups * 1e6 * BYTES_PER_LU[storage] / 1e9}
+        lat.close()
+        torch.cuda.empty_cache()
+
+    # accuracy of the shifted forms vs the FP64 run (reference arithmetic) at acc-steps
+    if args.acc_steps > 0:
+        macros = {}
+        for storage in ("f64", "f32s", "f16s"):
+            lat = P.Lattice(n, n, 2, storage)
+            lat.set_stream(stream)
+            lat.set_params(omega, MAGIC)
+            lat.equilibrium_init_dev(0, d_rho.data_ptr(), d_ux.data_ptr(), d_uy.data_ptr())
+            iold, _ = lat.run("trt", args.acc_steps, 0, 1)
+            m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
+            lat.macros_dev(iold, *(t.data_ptr() for t in m))
+            torch.cuda.synchronize()
+            macros[storage] = m
+            lat.close()
+        r64, u64, v64 = macros["f64"]
+        den = torch.sum(u64 * u64 + v64 * v64)
+        m64 = torch.sum(r64)
+        for storage in ("f32s", "f16s"):
+            r, u, v = macros[storage]
+            err = torch.sqrt(torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
+            results[storage]["vel_rel_l2_vs_f64"] = err
+            results[storage]["mass_rel_diff_vs_f64"] = abs((torch.sum(r) - m64) / m64).item()
+            results[storage]["acc_steps"] = args.acc_steps
+        results["f64"]["mass_drift_rel"] = abs((m64 - torch.sum(d_rho)) / torch.sum(d_rho)).item()
+        del macros
+        torch.cuda.empty_cache()
+
+    copy_gbs = P.copy_bandwidth(4 << 30, 20)
+    return results, copy_gbs, dev_name
+
+
+def cpu_leg(args):
+    """The reference Fortran (oracle/_ref, flang -O2 -fopenmp) on the host cores, over a
+    bounded sample of the same workload: the 8192^2 TRT grid for --cpu-steps steps."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    import __graft_entry__ as g
+    P = g.load_package()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    n = args.n
+    rho, ux, uy = initial_fields(P, n)
+    if po.have_ref():
+        R = po.RefLib()
+        grid = R.grid(n, n, "trt")
+        grid.set_properties(NU, 1.0, MAGIC)
+        grid.set_macros(rho, ux, uy)
+        grid.set_pdf_to_equilibrium()
+        grid.step(1)  # warm-up
+        t0 = time.perf_counter()
+        grid.step(args.cpu_steps)
+        dt = time.perf_counter() - t0
+        grid.close()
+        kind = "reference"
+    else:
+        O = po.Oracle()
+        fa, fb = po.new_f(n, n), po.new_f(n, n)
+        O.set_pdf_to_equilibrium(n, n, rho, ux, uy, fa)
+        omega = 1.0 / (3.0 * NU + 0.5)
+        O.run("trt", n, n, fa, fb, 1, omega, MAGIC)
+        t0 = time.perf_counter()
+        O.run("trt", n, n, fa, fb, args.cpu_steps, omega, MAGIC)
+        dt = time.perf_counter() - t0
+        kind = "port"
+    mlups = n * n * args.cpu_steps / dt / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": kind,
+            "sample": f"D2Q9 TRT {n}x{n} FP64, {args.cpu_steps} steps after 1 warm-up step "
+                      f"({'flang-built reference Fortran' if kind == 'reference' else 'C oracle'}, "
+                      f"OpenMP {cores} threads), {dt:.1f} s"}
+
+
+def traffic_from_profiles(n):
+    """HBM bytes per launch of the FP64 TRT kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    p = ROOT / "profiles" / f"traffic_trt_f64_{n}.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU slab decomposition is design-only", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+    results, copy_gbs, dev_name = gpu_leg(args)
+    cpu = None if args.no_cpu else cpu_leg(args)
+
+    f64 = results["f64"]
+    n = args.n
+    bytes_per_launch = BYTES_PER_LU["f64"] * n * n
+    achieved = bytes_per_launch / (f64["ms_per_step"] * 1e-3) / 1e9
+    line = {
+        "metric": "MLUPS & % HBM roofline, D2Q9 TRT 8192^2: FP64 vs FP32/FP16-shifted at stated err",
+        "value": round(f64["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green u0=0.05 nu=0.01 + U(+-1e-3) noise, seed 7)",
+        "config": {"workload": f"D2Q9 TRT Lambda=1/4 periodic {n}x{n}, FP64 populations, "
+                               "fused pull-stream+collide", "grid": [n, n], "model": "trt",
+                   "storage": "f64", "omega": 1.0 / (3.0 * NU + 0.5), "device": dev_name},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(n)},
+        "cpu_baseline": cpu,
+        "measured_copy_gbs": round(copy_gbs, 1),
+        "frac_of_measured_copy": round(achieved / copy_gbs, 4),
+        "storage_forms": {
+            k: {kk: (round(vv, 6) if isinstance(vv, float) and vv > 1e-3 else vv)
+                for kk, vv in v.items()} for k, v in results.items()},
+        "f32s_speedup_vs_f64": round(results["f32s"]["mlups"] / f64["mlups"], 3),
+        "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

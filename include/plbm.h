@@ -1,0 +1,174 @@
+/* plbm.h — C ABI of the MI355X (gfx950) D2Q9 periodic lattice Boltzmann engine.
+ *
+ * The hot path of ivan-pi/periodic-lbm is the Fortran module API
+ *   fvm_bardow / periodic_lbm / collision_bgk / collision_trt / collision_regularized
+ * (lattice container, equilibrium, stream + collide step, macroscopic moments).  This
+ * library is what that API binds to: the Fortran modules under periodic-lbm_amd/fortran/
+ * call these functions through ISO_C_BINDING, and the Python harness calls them through
+ * ctypes.  Each entry point names the reference routine it replaces (file:line).
+ *
+ * Conventions
+ *  - Every function returns an int status: PLBM_OK (0) or a PLBM_ERR_* code; the text of
+ *    the last error on the calling thread is returned by plbm_last_error().
+ *  - Host population arrays are the reference's f(ld, nx, 0:8) of one buffer, Fortran
+ *    order, unshifted FP64, with the caller's leading dimension ld >= ny:
+ *    element (y, x, q) (0-based) at f[(q*nx + x)*ld + y].
+ *  - Host macroscopic arrays are (ny, nx) Fortran order: element (y, x) at a[x*ny + y].
+ *  - Buffer ids are 0-based (the Fortran shim maps grid%iold/inew/imid 1-based ids).
+ *  - Entry points with the _dev suffix take device pointers (hipMalloc'd, or a torch
+ *    tensor's data_ptr) with the same layouts; they are asynchronous on the lattice's
+ *    stream.  Host-pointer entry points are synchronous.
+ *  - A lattice is not thread-safe; calls on one lattice must not overlap.
+ */
+#ifndef PLBM_H
+#define PLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLBM_OK 0
+#define PLBM_ERR_ARG 1      /* invalid argument (size, id, null pointer, model) */
+#define PLBM_ERR_HIP 2      /* a HIP runtime call failed (message has hipGetErrorString) */
+#define PLBM_ERR_STATE 3    /* call not valid in the current state (e.g. no params set) */
+#define PLBM_ERR_NOMEM 4    /* device allocation failed */
+#define PLBM_ERR_NUMERIC 5  /* non-finite populations detected by a checked call */
+#define PLBM_ERR_TIMEOUT 6  /* a bounded device-side wait gave up (batched engine) */
+
+/* Collision models: collide_bgk (src/collision_bgk.F90:17), collide_trt
+ * (src/collision_trt.F90:41), collide_rr (src/collision_regularized.F90:21). */
+#define PLBM_BGK 0
+#define PLBM_TRT 1
+#define PLBM_RR 2
+
+/* Population storage.  F64 reproduces the reference arithmetic bit for bit.  The
+ * shifted forms store g_q = f_q - w_q*rho0 (rho0 = 1): F64S in FP64 with FP64 math,
+ * F32S in FP32 and F16S in FP16 (scaled by a power of two) with FP32 math. */
+#define PLBM_F64 0
+#define PLBM_F64S 1
+#define PLBM_F32S 2
+#define PLBM_F16S 3
+
+/* Reduction slots written by plbm_reduce / plbm_reduce_dev (all FP64, deterministic). */
+#define PLBM_RED_MASS 0   /* sum rho                                  */
+#define PLBM_RED_KE 1     /* sum rho*(ux^2+uy^2)/2                     */
+#define PLBM_RED_ERR2 2   /* sum (ux-uxa)^2 + (uy-uya)^2 (0 if no ua)  */
+#define PLBM_RED_REF2 3   /* sum uxa^2 + uya^2 (0 if no ua)            */
+#define PLBM_RED_UMAX 4   /* max hypot(ux, uy)                         */
+#define PLBM_RED_NONFINITE 5 /* number of cells with a non-finite rho/u */
+#define PLBM_RED_U2 6     /* sum ux^2 + uy^2                           */
+#define PLBM_NRED 8
+
+typedef struct plbm_lattice plbm_lattice;
+
+typedef struct plbm_info {
+    int nx, ny;        /* grid */
+    int ldd;           /* device leading dimension (elements), >= ny */
+    int nf;            /* number of population buffers */
+    int storage;       /* PLBM_F64 ... PLBM_F16S */
+    int device;        /* HIP device ordinal */
+    double omega;      /* collision parameters last set */
+    double trt_magic;
+    double lambda_odd; /* lambda_d(omega, trt_magic), src/collision_trt.F90:30-34 */
+    double f16_scale;  /* storage scale for PLBM_F16S (power of two), 1 otherwise */
+    size_t bytes_per_buffer;
+} plbm_info;
+
+/* ---------------------------------------------------------------- lifecycle */
+
+/* alloc_grid (src/fvm_bardow.F90:129-202): device storage for nf population buffers of
+ * an nx x ny periodic grid, on the current HIP device, with its own stream. */
+int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out);
+/* dealloc_grid (src/fvm_bardow.F90:204-240). */
+int plbm_destroy(plbm_lattice *h);
+int plbm_get_info(const plbm_lattice *h, plbm_info *out);
+/* Run this lattice's work on `stream` (a hipStream_t; NULL = the lattice's own). */
+int plbm_set_stream(plbm_lattice *h, void *stream);
+/* Device pointer of population buffer `buf` (layout [q][x][ldd], storage type). */
+int plbm_buffer_ptr(plbm_lattice *h, int buf, void **dptr);
+
+/* set_properties (src/fvm_bardow.F90:242-269) computes omega and trt_magic on the host
+ * side; this hands them to the device path.  lambda_odd = lambda_d(omega, trt_magic). */
+int plbm_set_params(plbm_lattice *h, double omega, double trt_magic);
+/* Override the FP16 storage scale (power of two, default 1024); set before loading. */
+int plbm_set_f16_scale(plbm_lattice *h, double scale);
+
+/* ---------------------------------------------------------------- populations */
+
+/* Copy one buffer from/to the host array f(ld, nx, 0:8) (unshifted FP64). */
+int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld);
+int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld);
+
+/* set_pdf_to_equilibrium (src/fvm_bardow.F90:272-305) with equilibrium()
+ * (src/fvm_bardow.F90:99-126): buffer buf = feq(rho, ux, uy) per cell. */
+int plbm_equilibrium_init(plbm_lattice *h, int buf, const double *rho, const double *ux,
+                          const double *uy);
+int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const double *ux,
+                              const double *uy);
+
+/* ---------------------------------------------------------------- stepping */
+
+/* One fused pull-stream + collide step, src -> dst: lbm_stream
+ * (src/periodic_lbm.f90:32-129) followed by the collision on the streamed buffer
+ * (src/collision_*.F90), in one pass over HBM. */
+int plbm_step(plbm_lattice *h, int model, int src, int dst);
+/* perform_lbm_step (src/periodic_lbm.f90:15-29) nsteps times with the reference's
+ * buffer bookkeeping: on entry *iold / *inew are the current ids; each step reads *iold,
+ * writes *inew, then swaps them.  Asynchronous. */
+int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew);
+/* Unfused operators for the procedure-pointer slots grid%streaming / grid%collision:
+ * lbm_stream (src/periodic_lbm.f90:32-129) src -> dst; collide_* in place on buf. */
+int plbm_stream(plbm_lattice *h, int src, int dst);
+int plbm_collide(plbm_lattice *h, int model, int buf);
+/* Buffer copy dst = src (perform_triple_step's pre-collision store,
+ * src/fvm_bardow.F90:328). */
+int plbm_copy(plbm_lattice *h, int src, int dst);
+
+/* ---------------------------------------------------------------- read-back */
+
+/* update_macros (src/fvm_bardow.F90:343-390) on buffer buf: rho and u = j/rho, in the
+ * reference's summation order; (ny, nx) Fortran-order outputs. */
+int plbm_macros(plbm_lattice *h, int buf, double *rho, double *ux, double *uy);
+int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *uy);
+
+/* FP64 reductions of buffer buf (slots PLBM_RED_*): mass, kinetic energy, max|u|,
+ * non-finite count and, when uxa/uya are non-NULL ((ny,nx) analytic velocity), the
+ * sums behind the app's relative L2 norm (app/main_taylor_green.f90:174-212). */
+int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                double out[PLBM_NRED]);
+int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                    double out[PLBM_NRED]);
+
+int plbm_sync(plbm_lattice *h);
+const char *plbm_last_error(void);
+
+/* ---------------------------------------------------------------- flow cases (host) */
+
+/* taylor_green_t%eval (src/benchmarks/taylor_green.f90:31-84) on an (ny, nx) grid. */
+int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, double nu,
+                            double t, double *p, double *ux, double *uy);
+/* shear_layer_t%eval (src/benchmarks/shear_layer.f90:123-150). */
+int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
+                           double *rho, double *ux, double *uy);
+/* ux += amp*(2U-1), uy += amp*(2U'-1) with U, U' the counter-based deviates of cell c
+ * (counters 2c and 2c+1) of `seed`; the generator is splitmix64's output function. */
+int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy);
+double plbm_noise(uint64_t seed, uint64_t counter);
+
+/* lambda_d / magic_number (src/collision_trt.F90:24-34). */
+double plbm_lambda_d(double omega, double x);
+double plbm_magic_number(double le, double ld);
+
+/* ---------------------------------------------------------------- measurement */
+
+/* HBM probe: time `iters` launches of a streaming copy of `bytes` bytes (read + write)
+ * on the current device; returns the mean kernel time in milliseconds via *ms. */
+int plbm_copy_bandwidth(size_t bytes, int iters, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLBM_H */

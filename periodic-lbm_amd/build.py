@@ -1,0 +1,144 @@
+"""Build recipe for the engine (run by __graft_entry__.build(); no cmake, no JIT cache).
+
+Outputs, all in-tree so they travel with the source tree:
+  periodic-lbm_amd/libplbm.so          HIP kernels + C ABI (include/plbm.h), gfx950
+  periodic-lbm_amd/fortran/_build/     Fortran shim: module files + libplbm_fortran.a
+  periodic-lbm_amd/_build/app/         the reference's app/main_taylor_green.f90, compiled
+                                       unmodified against the shim (only where the
+                                       reference tree is present at build time)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+FSRC = PKG / "fortran"
+OBJ = PKG / "_build" / "obj"
+FBUILD = FSRC / "_build"
+APPDIR = PKG / "_build" / "app"
+LIB = PKG / "libplbm.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["lattice.hip"]
+CXX_SOURCES = ["fields.cpp"]
+# -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
+# reference is built without FMA); kernels that want FMA enable it locally by pragma.
+# Packed FP32 instructions are disabled for all device code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", f"-I{ROOT / 'include'}",
+]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-Wall",
+             f"-I{ROOT / 'include'}"]
+# Fortran shim modules in dependency order
+FORTRAN_SOURCES = [
+    "plbm_capi.f90", "precision.F90", "fvm_bardow.F90", "periodic_lbm.f90",
+    "collision_bgk.f90", "collision_trt.f90", "collision_regularized.f90",
+    "taylor_green.f90", "shear_layer.f90",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run([str(c) for c in cmd], cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(map(str, cmd))}\n"
+                           f"{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build_hip(verbose: bool = True) -> Path:
+    OBJ.mkdir(parents=True, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", CSRC / s, "-o", OBJ / (Path(s).stem + ".o")])
+    for s in CXX_SOURCES:
+        jobs.append(["g++", *CXX_FLAGS, "-c", CSRC / s, "-o", OBJ / (Path(s).stem + ".o")])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    objs = [OBJ / (Path(s).stem + ".o") for s in HIP_SOURCES + CXX_SOURCES]
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"[build] {LIB.relative_to(ROOT)}")
+    return LIB
+
+
+def build_fortran(verbose: bool = True) -> Path:
+    """Compile the Fortran module shim (ISO_C_BINDING over libplbm.so)."""
+    mod = FBUILD / "mod"
+    mod.mkdir(parents=True, exist_ok=True)
+    objs = []
+    for s in FORTRAN_SOURCES:
+        o = FBUILD / (Path(s).stem + ".o")
+        _run([FLANG, "-O2", "-fPIC", "-fopenmp", "-module-dir", mod, f"-I{mod}", "-c",
+              FSRC / s, "-o", o])
+        objs.append(o)
+    lib = FBUILD / "libplbm_fortran.a"
+    if lib.exists():
+        lib.unlink()
+    _run(["ar", "rcs", lib, *objs])
+    if verbose:
+        print(f"[build] {lib.relative_to(ROOT)}")
+    return lib
+
+
+def link_fortran_program(src: Path, out: Path, moddir: Path | None = None) -> Path:
+    """Compile a Fortran program that `use`s the reference module names against the
+    shim and libplbm.so.  The internal-procedure logger of the reference app needs an
+    executable stack for its trampoline."""
+    mod = FBUILD / "mod"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    pmod = moddir or (out.parent / "mod")
+    pmod.mkdir(parents=True, exist_ok=True)
+    _run([FLANG, "-O2", "-fopenmp", f"-I{mod}", "-module-dir", pmod, src,
+          FBUILD / "libplbm_fortran.a", f"-L{PKG}", "-lplbm", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../..", "-Wl,-z,execstack", "-o", out])
+    return out
+
+
+def build_app(ref_root: Path, verbose: bool = True) -> Path | None:
+    src = ref_root / "app" / "main_taylor_green.f90"
+    if not src.exists():
+        return None
+    out = link_fortran_program(src, APPDIR / "main_taylor_green")
+    if verbose:
+        print(f"[build] {out.relative_to(ROOT)} (unchanged {src.name} on the HIP engine)")
+    return out
+
+
+def build_oracle(ref_root: Path | None, verbose: bool = True) -> None:
+    _run(["make", "-C", ROOT / "oracle", "liboracle"])
+    if verbose:
+        print("[build] oracle/liboracle.so")
+    if ref_root is not None and (ref_root / "src" / "fvm_bardow.F90").exists():
+        _run(["make", "-C", ROOT / "oracle", "ref", f"REF={ref_root}"])
+        if verbose:
+            print("[build] oracle/_ref (reference Fortran, flang)")
+
+
+def build_all(verbose: bool = True) -> None:
+    ref = Path(os.environ.get("PLBM_REFERENCE", "/root/reference"))
+    ref_root = ref if (ref / "src").exists() else None
+    build_hip(verbose)
+    if shutil.which(FLANG) or Path(FLANG).exists():
+        build_fortran(verbose)
+        if ref_root is not None:
+            build_app(ref_root, verbose)
+    build_oracle(ref_root, verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,80 @@
+// fields.cpp — host-side initial and analytic fields (flow cases), FP64.
+//
+// Built with -ffp-contract=off: each expression follows the reference's evaluation
+// order so the fields equal the reference's bit for bit when both call the same libm
+// (glibc sqrt/sin/cos/exp/tanh).
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/plbm.h"
+
+extern "C" {
+
+// taylor_green_t constructor + eval, src/benchmarks/taylor_green.f90:31-84.
+// Cell centres are (x-1)+1/2 (:67,69).  The reference's inner loop runs y = 1..nx (:68),
+// which is the same as 1..ny on the square grids it is used with; here y runs over ny.
+int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, double nu,
+                            double t, double *p, double *ux, double *uy) {
+    if (nx < 1 || ny < 1 || !p || !ux || !uy || !(kx > 0) || !(ky > 0) || !(nu > 0))
+        return PLBM_ERR_ARG;
+    const double td = 1.0 / (nu * (kx * kx + ky * ky));
+    const double sx = std::sqrt(ky / kx), sy = std::sqrt(kx / ky);
+    const double ex = std::exp(-(t / td)), ep = std::exp(-((2.0 * t) / td));
+    const double pa = 0.25 * (umax * umax), rk = ky / kx, rk2 = kx / ky;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        const double xx = double(x) + 0.5;
+        const double cxx = std::cos(kx * xx), sxx = std::sin(kx * xx);
+        const double c2x = std::cos((2.0 * kx) * xx);
+        for (int y = 0; y < ny; ++y) {
+            const double yy = double(y) + 0.5;
+            const size_t c = (size_t)x * ny + y;
+            ux[c] = -((((umax * sx) * cxx) * std::sin(ky * yy)) * ex);
+            uy[c] = (((umax * sy) * sxx) * std::cos(ky * yy)) * ex;
+            p[c] = -((pa * (rk * c2x + rk2 * std::cos((2.0 * ky) * yy))) * ep);
+        }
+    }
+    return PLBM_OK;
+}
+
+// eval_shear_layer_grid, src/benchmarks/shear_layer.f90:123-150.
+int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
+                           double *rho, double *ux, double *uy) {
+    if (nx < 1 || ny < 1 || !rho || !ux || !uy || !(length > 0)) return PLBM_ERR_ARG;
+    const double pi = 4.0 * std::atan(1.0);
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        const double xd = (double(x) + 0.5) / length;
+        const double v = (u0 * delta) * std::sin((2.0 * pi) * (xd + 0.25));
+        for (int y = 0; y < ny; ++y) {
+            const double yd = (double(y) + 0.5) / length;
+            const size_t c = (size_t)x * ny + y;
+            uy[c] = v;
+            ux[c] = yd <= 0.5 ? u0 * std::tanh(k * (yd - 0.25)) : u0 * std::tanh(k * (0.75 - yd));
+            rho[c] = 1.0;
+        }
+    }
+    return PLBM_OK;
+}
+
+// Counter-based uniform deviate in [0,1): splitmix64's output function applied to
+// seed + (counter+1)*0x9E3779B97F4A7C15, top 53 bits.
+double plbm_noise(uint64_t seed, uint64_t counter) {
+    uint64_t z = seed + (counter + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy) {
+    if (n < 0 || !ux || !uy) return PLBM_ERR_ARG;
+    #pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        ux[c] += amp * (2.0 * plbm_noise(seed, 2 * (uint64_t)c) - 1.0);
+        uy[c] += amp * (2.0 * plbm_noise(seed, 2 * (uint64_t)c + 1) - 1.0);
+    }
+    return PLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,752 @@
+// lattice.hip — the single-lattice engine: device kernels and the plbm_* C ABI.
+//
+// HBM layout of one population buffer: nine q-planes, each nx columns of ldd elements,
+// y unit-stride — the reference's f(ld, nx, 0:8) (src/fvm_bardow.F90:153) with the
+// device pitch ldd = ny rounded up to 64 elements so every column starts 128-byte
+// aligned for FP16 and 512-byte aligned for FP64.
+//
+// Hot path: k_fused, one pull-stream + collide pass per step (lbm_stream,
+// src/periodic_lbm.f90:45-127, then collide_*, src/collision_*.F90).  Pull streaming
+// is a permutation, so fusing it with the collision gives the same numbers as the
+// reference's two passes while touching HBM once: 9 reads + 9 writes per cell.
+// Periodic wrap is a select on the neighbour index; there is no halo.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "d2q9.hpp"
+
+namespace plbm {
+
+// ---------------------------------------------------------------------------- errors
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------- helpers
+
+// Workgroup b of G is dispatched to XCD b % 8.  Give each XCD a contiguous range of
+// logical tiles so neighbouring y-tiles of a column (which share the cache line that a
+// y+-1 shifted load straddles) sit in the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    constexpr int NX = 8;
+    const int q = G / NX, r = G % NX;
+    const int xcd = b % NX, idx = b / NX;
+    const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+    return base + idx;
+}
+
+template <int ST>
+__device__ __forceinline__ typename Traits<ST>::R load_pop(const typename Traits<ST>::T *p,
+                                                           float inv_scale) {
+    using R = typename Traits<ST>::R;
+    if constexpr (ST == PLBM_F16S)
+        return R(*p) * inv_scale;
+    else
+        return R(*p);
+}
+
+template <int ST>
+__device__ __forceinline__ void store_pop(typename Traits<ST>::T *p, typename Traits<ST>::R v,
+                                          float scale) {
+    using T = typename Traits<ST>::T;
+    if constexpr (ST == PLBM_F16S)
+        __builtin_nontemporal_store(T(v * scale), p);
+    else
+        __builtin_nontemporal_store(T(v), p);
+}
+
+template <int MODEL, int ST>
+__device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const StepArgs &a) {
+    using R = typename Traits<ST>::R;
+    if constexpr (!Traits<ST>::shifted) {
+        if constexpr (MODEL == PLBM_BGK) collide_bgk_ref(f, a.omega);
+        else if constexpr (MODEL == PLBM_TRT) collide_trt_ref(f, a.omega, a.lambda_odd);
+        else collide_rr_ref(f, a.omega);
+    } else {
+        const R om = sizeof(R) == 8 ? R(a.omega) : R(a.omega_f);
+        const R lo = sizeof(R) == 8 ? R(a.lambda_odd) : R(a.lambda_odd_f);
+        if constexpr (MODEL == PLBM_BGK) collide_bgk_shifted<R>(f, om);
+        else if constexpr (MODEL == PLBM_TRT) collide_trt_shifted<R>(f, om, lo);
+        else collide_rr_shifted<R>(f, om);
+    }
+}
+
+// ---------------------------------------------------------------------------- kernels
+
+// Fused pull-stream + collide.  One thread per cell; a workgroup covers blockDim.x
+// consecutive y of one column x.  f_q(y, x) <- src_q(y - cy_q, x - cx_q).
+template <int MODEL, int ST, bool REMAP>
+__global__ void __launch_bounds__(256) k_fused(StepArgs a) {
+    using T = typename Traits<ST>::T;
+    using R = typename Traits<ST>::R;
+    const int b = REMAP ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int by = b % a.nby, x = b / a.nby;
+    const int y = by * blockDim.x + threadIdx.x;
+    if (y >= a.ny) return;
+    const int nx = a.nx, ny = a.ny;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const size_t P = a.plane, L = (size_t)a.ldd;
+    const size_t c0 = (size_t)x * L, cm = (size_t)xm * L, cp = (size_t)xp * L;
+    const T *__restrict__ s = static_cast<const T *>(a.src);
+    T *__restrict__ d = static_cast<T *>(a.dst);
+    const float is = a.inv_scale;
+    R f[9];
+    f[0] = load_pop<ST>(s + 0 * P + c0 + y, is);
+    f[1] = load_pop<ST>(s + 1 * P + cm + y, is);
+    f[2] = load_pop<ST>(s + 2 * P + c0 + ym, is);
+    f[3] = load_pop<ST>(s + 3 * P + cp + y, is);
+    f[4] = load_pop<ST>(s + 4 * P + c0 + yp, is);
+    f[5] = load_pop<ST>(s + 5 * P + cm + ym, is);
+    f[6] = load_pop<ST>(s + 6 * P + cp + ym, is);
+    f[7] = load_pop<ST>(s + 7 * P + cp + yp, is);
+    f[8] = load_pop<ST>(s + 8 * P + cm + yp, is);
+    collide_cell<MODEL, ST>(f, a);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * P + c0 + y, f[q], a.scale);
+}
+
+// lbm_stream alone (src/periodic_lbm.f90:45-127): raw copy of stored values.
+template <class T>
+__global__ void __launch_bounds__(256) k_stream(const T *__restrict__ s, T *__restrict__ d,
+                                                int nx, int ny, int ldd, size_t P) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= ny) return;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int xs = (x - cx(q) + nx) % nx, ys = (y - cy(q) + ny) % ny;
+        d[q * P + (size_t)x * ldd + y] = s[q * P + (size_t)xs * ldd + ys];
+    }
+}
+
+// collide_* alone, in place on one buffer.
+template <int MODEL, int ST>
+__global__ void __launch_bounds__(256) k_collide(StepArgs a) {
+    using T = typename Traits<ST>::T;
+    using R = typename Traits<ST>::R;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= a.ny) return;
+    T *d = static_cast<T *>(a.dst);
+    const size_t c = (size_t)x * a.ldd + y;
+    R f[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = load_pop<ST>(d + q * a.plane + c, a.inv_scale);
+    collide_cell<MODEL, ST>(f, a);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * a.plane + c, f[q], a.scale);
+}
+
+// Cell macros in FP64.  F64: update_macros order (src/fvm_bardow.F90:371-379).
+// Shifted: rho = 1 + sum g, u = j/rho.
+template <int ST>
+__device__ __forceinline__ void cell_macros(const typename Traits<ST>::T *s, size_t P, size_t c,
+                                            float inv_scale, double &rho, double &ux,
+                                            double &uy) {
+    double f[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = double(load_pop<ST>(s + q * P + c, inv_scale));
+    if constexpr (!Traits<ST>::shifted) {
+        macros_ref(f, rho, ux, uy);
+    } else {
+        double drho, jx, jy;
+        moments_shifted<double>(f, drho, jx, jy);
+        rho = 1.0 + drho;
+        ux = jx / rho;
+        uy = jy / rho;
+    }
+}
+
+template <int ST>
+__global__ void __launch_bounds__(256) k_macros(const void *src, int nx, int ny, int ldd,
+                                                size_t P, float inv_scale, double *rho,
+                                                double *ux, double *uy) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= ny) return;
+    const auto *s = static_cast<const typename Traits<ST>::T *>(src);
+    double r, u, v;
+    cell_macros<ST>(s, P, (size_t)x * ldd + y, inv_scale, r, u, v);
+    const size_t o = (size_t)x * ny + y;
+    rho[o] = r;
+    ux[o] = u;
+    uy[o] = v;
+}
+
+// Deterministic two-pass FP64 reduction: fixed cell->thread map (grid-stride over the
+// column-major cell index), fixed LDS tree per block, one block for the final pass.
+template <int ST>
+__global__ void __launch_bounds__(256) k_reduce(const void *src, int nx, int ny, int ldd,
+                                                size_t P, float inv_scale, const double *uxa,
+                                                const double *uya, double *partials) {
+    __shared__ double sh[256][PLBM_NRED];
+    const auto *s = static_cast<const typename Traits<ST>::T *>(src);
+    double acc[PLBM_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const size_t n = (size_t)nx * ny;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i / ny), y = (int)(i % ny);
+        double rho, ux, uy;
+        cell_macros<ST>(s, P, (size_t)x * ldd + y, inv_scale, rho, ux, uy);
+        const double u2 = ux * ux + uy * uy;
+        if (!(__builtin_isfinite(rho) && __builtin_isfinite(ux) && __builtin_isfinite(uy))) {
+            acc[PLBM_RED_NONFINITE] += 1.0;
+            continue;
+        }
+        acc[PLBM_RED_MASS] += rho;
+        acc[PLBM_RED_KE] += 0.5 * rho * u2;
+        acc[PLBM_RED_U2] += u2;
+        acc[PLBM_RED_UMAX] = fmax(acc[PLBM_RED_UMAX], sqrt(u2));
+        if (uxa) {
+            const double ex = ux - uxa[i], ey = uy - uya[i];
+            acc[PLBM_RED_ERR2] += ex * ex + ey * ey;
+            acc[PLBM_RED_REF2] += uxa[i] * uxa[i] + uya[i] * uya[i];
+        }
+    }
+    for (int k = 0; k < PLBM_NRED; ++k) sh[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride) {
+            for (int k = 0; k < PLBM_NRED; ++k) {
+                const double o = sh[threadIdx.x + stride][k];
+                sh[threadIdx.x][k] = k == PLBM_RED_UMAX ? fmax(sh[threadIdx.x][k], o)
+                                                         : sh[threadIdx.x][k] + o;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < PLBM_NRED) partials[blockIdx.x * PLBM_NRED + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+__global__ void __launch_bounds__(PLBM_NRED) k_reduce_final(const double *partials, int nblocks,
+                                                            double *out) {
+    const int k = threadIdx.x;
+    double a = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+        const double v = partials[b * PLBM_NRED + k];
+        a = k == PLBM_RED_UMAX ? fmax(a, v) : a + v;
+    }
+    out[k] = a;
+}
+
+// set_pdf_to_equilibrium: buffer = feq(rho, u).  Shifted forms are evaluated in FP64 and
+// rounded once to the storage type.
+template <int ST>
+__global__ void __launch_bounds__(256) k_equilibrium(const double *rho, const double *ux,
+                                                     const double *uy, void *dst, int nx,
+                                                     int ny, int ldd, size_t P, float scale) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= ny) return;
+    auto *d = static_cast<typename Traits<ST>::T *>(dst);
+    const size_t i = (size_t)x * ny + y, c = (size_t)x * ldd + y;
+    double feq[9];
+    if constexpr (!Traits<ST>::shifted)
+        equilibrium_ref(rho[i], ux[i], uy[i], feq);
+    else
+        equilibrium_shifted<double>(rho[i] - 1.0, ux[i], uy[i], feq);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * P + c, feq[q], scale);
+}
+
+// One q-plane between host layout (FP64, unshifted, pitch ld) and storage.
+template <int ST>
+__global__ void __launch_bounds__(256) k_plane_from_f64(const double *src, int ld, void *dst,
+                                                        int nx, int ny, int ldd, double wshift,
+                                                        float scale) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= ny) return;
+    auto *d = static_cast<typename Traits<ST>::T *>(dst);
+    const double v = src[(size_t)x * ld + y];
+    using T = typename Traits<ST>::T;
+    if constexpr (ST == PLBM_F16S)
+        d[(size_t)x * ldd + y] = T(float((v - wshift) * double(scale)));
+    else if constexpr (Traits<ST>::shifted)
+        d[(size_t)x * ldd + y] = T(v - wshift);
+    else
+        d[(size_t)x * ldd + y] = T(v);
+}
+
+template <int ST>
+__global__ void __launch_bounds__(256) k_plane_to_f64(const void *src, int nx, int ny, int ldd,
+                                                      double *dst, int ld, double wshift,
+                                                      float inv_scale) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    if (y >= ny) return;
+    const auto *s = static_cast<const typename Traits<ST>::T *>(src);
+    const double v = double(s[(size_t)x * ldd + y]);
+    if constexpr (ST == PLBM_F16S)
+        dst[(size_t)x * ld + y] = v * double(inv_scale) + wshift;
+    else if constexpr (Traits<ST>::shifted)
+        dst[(size_t)x * ld + y] = v + wshift;
+    else
+        dst[(size_t)x * ld + y] = v;
+}
+
+// HBM probe: 16-byte vector copy, grid-stride.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy(const v4f *__restrict__ s, v4f *__restrict__ d,
+                                              size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(s[i], d + i);
+}
+
+// ---------------------------------------------------------------------------- dispatch
+
+template <class F>
+static int dispatch_storage(int st, F &&f) {
+    switch (st) {
+    case PLBM_F64: return f(std::integral_constant<int, PLBM_F64>{});
+    case PLBM_F64S: return f(std::integral_constant<int, PLBM_F64S>{});
+    case PLBM_F32S: return f(std::integral_constant<int, PLBM_F32S>{});
+    case PLBM_F16S: return f(std::integral_constant<int, PLBM_F16S>{});
+    }
+    set_error("unknown storage id " + std::to_string(st));
+    return PLBM_ERR_ARG;
+}
+
+template <class F>
+static int dispatch_model(int m, F &&f) {
+    switch (m) {
+    case PLBM_BGK: return f(std::integral_constant<int, PLBM_BGK>{});
+    case PLBM_TRT: return f(std::integral_constant<int, PLBM_TRT>{});
+    case PLBM_RR: return f(std::integral_constant<int, PLBM_RR>{});
+    }
+    set_error("unknown collision model " + std::to_string(m));
+    return PLBM_ERR_ARG;
+}
+
+static StepArgs make_args(const plbm_lattice *h, const void *src, void *dst) {
+    StepArgs a;
+    a.src = src;
+    a.dst = dst;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.ldd = h->ldd;
+    a.nby = (h->ny + h->block - 1) / h->block;
+    a.plane = h->plane;
+    a.omega = h->omega;
+    a.lambda_odd = h->lambda_odd;
+    a.omega_f = float(h->omega);
+    a.lambda_odd_f = float(h->lambda_odd);
+    a.scale = float(h->scale);
+    a.inv_scale = float(1.0 / h->scale);
+    return a;
+}
+
+static int check_buf(const plbm_lattice *h, int buf) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(buf >= 0 && buf < h->nf, PLBM_ERR_ARG,
+                 "buffer id " + std::to_string(buf) + " out of range [0," +
+                     std::to_string(h->nf) + ")");
+    return PLBM_OK;
+}
+
+static int ensure_tmp(plbm_lattice *h, size_t bytes) {
+    if (h->tmp_bytes >= bytes) return PLBM_OK;
+    if (h->d_tmp) PLBM_HIP(hipFree(h->d_tmp));
+    h->d_tmp = nullptr;
+    h->tmp_bytes = 0;
+    if (hipMalloc(&h->d_tmp, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc of " + std::to_string(bytes) + " staging bytes failed");
+        return PLBM_ERR_NOMEM;
+    }
+    h->tmp_bytes = bytes;
+    return PLBM_OK;
+}
+
+static dim3 cell_grid(const plbm_lattice *h) {
+    return dim3((unsigned)((h->ny + 255) / 256), (unsigned)h->nx);
+}
+
+int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
+    StepArgs a = make_args(h, src, dst);
+    const unsigned nblocks = (unsigned)((size_t)a.nby * h->nx);
+    return dispatch_model(model, [&](auto M) {
+        return dispatch_storage(h->storage, [&](auto S) {
+            if (h->xcd_remap)
+                hipLaunchKernelGGL((k_fused<decltype(M)::value, decltype(S)::value, true>),
+                                   dim3(nblocks), dim3(h->block), 0, h->stream, a);
+            else
+                hipLaunchKernelGGL((k_fused<decltype(M)::value, decltype(S)::value, false>),
+                                   dim3(nblocks), dim3(h->block), 0, h->stream, a);
+            PLBM_HIP(hipGetLastError());
+            return PLBM_OK;
+        });
+    });
+}
+
+static int require_params(const plbm_lattice *h) {
+    PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE,
+                 "collision parameters not set: call plbm_set_params first");
+    return PLBM_OK;
+}
+
+}  // namespace plbm
+
+using namespace plbm;
+
+// ============================================================================ C ABI
+
+extern "C" {
+
+const char *plbm_last_error(void) { return last_error(); }
+
+double plbm_lambda_d(double omega, double x) {
+    // src/collision_trt.F90:30-34
+    return (4.0 - 2.0 * omega) / ((((4.0 * x) * omega) + 2.0) - omega);
+}
+
+double plbm_magic_number(double le, double ld) {
+    // src/collision_trt.F90:24-28
+    return ((2.0 - le) * (2.0 - ld)) / ((4.0 * le) * ld);
+}
+
+int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) {
+    PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    PLBM_REQUIRE(nx >= 1 && ny >= 1, PLBM_ERR_ARG, "grid size must be positive");
+    PLBM_REQUIRE(nf >= 1 && nf <= kMaxBuffers, PLBM_ERR_ARG,
+                 "nf must be in [1," + std::to_string(kMaxBuffers) + "]");
+    PLBM_REQUIRE(storage >= PLBM_F64 && storage <= PLBM_F16S, PLBM_ERR_ARG, "unknown storage");
+    const int ldd = (ny + 63) / 64 * 64;
+    PLBM_REQUIRE((size_t)nx * ldd < ((size_t)1 << 31), PLBM_ERR_ARG,
+                 "plane of nx*ldd elements must stay below 2^31");
+    auto *h = new plbm_lattice();
+    h->nx = nx;
+    h->ny = ny;
+    h->ldd = ldd;
+    h->nf = nf;
+    h->storage = storage;
+    h->plane = (size_t)nx * ldd;
+    h->scale = storage == PLBM_F16S ? 1024.0 : 1.0;
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("no usable HIP device / stream");
+        delete h;
+        return PLBM_ERR_HIP;
+    }
+    h->stream = h->own;
+    const size_t bytes = 9 * h->plane * storage_bytes(storage);
+    for (int b = 0; b < nf; ++b) {
+        if (hipMalloc(&h->f[b], bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("hipMalloc of population buffer (" + std::to_string(bytes) +
+                      " bytes) failed");
+            plbm_destroy(h);
+            return PLBM_ERR_NOMEM;
+        }
+    }
+    *out = h;
+    return PLBM_OK;
+}
+
+int plbm_destroy(plbm_lattice *h) {
+    if (!h) return PLBM_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int b = 0; b < kMaxBuffers; ++b)
+        if (h->f[b]) (void)hipFree(h->f[b]);
+    if (h->d_tmp) (void)hipFree(h->d_tmp);
+    if (h->d_partials) (void)hipFree(h->d_partials);
+    if (h->d_red) (void)hipFree(h->d_red);
+    if (h->own) (void)hipStreamDestroy(h->own);
+    delete h;
+    return PLBM_OK;
+}
+
+int plbm_get_info(const plbm_lattice *h, plbm_info *out) {
+    PLBM_REQUIRE(h && out, PLBM_ERR_ARG, "null argument");
+    out->nx = h->nx;
+    out->ny = h->ny;
+    out->ldd = h->ldd;
+    out->nf = h->nf;
+    out->storage = h->storage;
+    out->device = h->device;
+    out->omega = h->omega;
+    out->trt_magic = h->trt_magic;
+    out->lambda_odd = h->lambda_odd;
+    out->f16_scale = h->scale;
+    out->bytes_per_buffer = 9 * h->plane * storage_bytes(h->storage);
+    return PLBM_OK;
+}
+
+int plbm_set_stream(plbm_lattice *h, void *stream) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own;
+    return PLBM_OK;
+}
+
+int plbm_buffer_ptr(plbm_lattice *h, int buf, void **dptr) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(dptr, PLBM_ERR_ARG, "null output pointer");
+    *dptr = h->f[buf];
+    return PLBM_OK;
+}
+
+int plbm_set_params(plbm_lattice *h, double omega, double trt_magic) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(std::isfinite(omega) && std::isfinite(trt_magic), PLBM_ERR_ARG,
+                 "non-finite collision parameter");
+    h->omega = omega;
+    h->trt_magic = trt_magic;
+    h->lambda_odd = plbm_lambda_d(omega, trt_magic);
+    h->params_set = true;
+    return PLBM_OK;
+}
+
+int plbm_set_f16_scale(plbm_lattice *h, double scale) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    int e = 0;
+    PLBM_REQUIRE(scale > 0 && std::frexp(scale, &e) == 0.5, PLBM_ERR_ARG,
+                 "FP16 scale must be a positive power of two");
+    h->scale = h->storage == PLBM_F16S ? scale : 1.0;
+    return PLBM_OK;
+}
+
+int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
+    const size_t plane_host = (size_t)ld * h->nx;
+    PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
+    const size_t eb = storage_bytes(h->storage);
+    for (int q = 0; q < 9; ++q) {
+        // host plane (pitch ld) -> compact staging plane (pitch ny)
+        PLBM_HIP(hipMemcpy2DAsync(h->d_tmp, (size_t)h->ny * 8, f + q * plane_host, (size_t)ld * 8,
+                                  (size_t)h->ny * 8, h->nx, hipMemcpyHostToDevice, h->stream));
+        void *dst = static_cast<char *>(h->f[buf]) + q * h->plane * eb;
+        const double w = wq(q);
+        PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
+            hipLaunchKernelGGL(k_plane_from_f64<decltype(S)::value>, cell_grid(h), dim3(256), 0,
+                               h->stream, h->d_tmp, h->ny, dst, h->nx, h->ny, h->ldd, w,
+                               float(h->scale));
+            PLBM_HIP(hipGetLastError());
+            return PLBM_OK;
+        }));
+        PLBM_HIP(hipStreamSynchronize(h->stream));
+    }
+    return PLBM_OK;
+}
+
+int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
+    const size_t plane_host = (size_t)ld * h->nx;
+    PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
+    const size_t eb = storage_bytes(h->storage);
+    for (int q = 0; q < 9; ++q) {
+        const void *src = static_cast<const char *>(h->f[buf]) + q * h->plane * eb;
+        const double w = wq(q);
+        PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
+            hipLaunchKernelGGL(k_plane_to_f64<decltype(S)::value>, cell_grid(h), dim3(256), 0,
+                               h->stream, src, h->nx, h->ny, h->ldd, h->d_tmp, h->ny, w,
+                               float(1.0 / h->scale));
+            PLBM_HIP(hipGetLastError());
+            return PLBM_OK;
+        }));
+        // compact staging plane -> host plane (pitch ld); padding rows ny..ld-1 untouched
+        PLBM_HIP(hipMemcpy2DAsync(f + q * plane_host, (size_t)ld * 8, h->d_tmp, (size_t)h->ny * 8,
+                                  (size_t)h->ny * 8, h->nx, hipMemcpyDeviceToHost, h->stream));
+        PLBM_HIP(hipStreamSynchronize(h->stream));
+    }
+    return PLBM_OK;
+}
+
+int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const double *ux,
+                              const double *uy) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
+    return dispatch_storage(h->storage, [&](auto S) {
+        hipLaunchKernelGGL(k_equilibrium<decltype(S)::value>, cell_grid(h), dim3(256), 0,
+                           h->stream, rho, ux, uy, h->f[buf], h->nx, h->ny, h->ldd, h->plane,
+                           float(h->scale));
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    });
+}
+
+int plbm_equilibrium_init(plbm_lattice *h, int buf, const double *rho, const double *ux,
+                          const double *uy) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
+    const size_t n = (size_t)h->nx * h->ny;
+    PLBM_TRY(ensure_tmp(h, 3 * n * sizeof(double)));
+    PLBM_HIP(hipMemcpyAsync(h->d_tmp, rho, n * 8, hipMemcpyHostToDevice, h->stream));
+    PLBM_HIP(hipMemcpyAsync(h->d_tmp + n, ux, n * 8, hipMemcpyHostToDevice, h->stream));
+    PLBM_HIP(hipMemcpyAsync(h->d_tmp + 2 * n, uy, n * 8, hipMemcpyHostToDevice, h->stream));
+    PLBM_TRY(plbm_equilibrium_init_dev(h, buf, h->d_tmp, h->d_tmp + n, h->d_tmp + 2 * n));
+    PLBM_HIP(hipStreamSynchronize(h->stream));
+    return PLBM_OK;
+}
+
+int plbm_step(plbm_lattice *h, int model, int src, int dst) {
+    PLBM_TRY(check_buf(h, src));
+    PLBM_TRY(check_buf(h, dst));
+    PLBM_REQUIRE(src != dst, PLBM_ERR_ARG, "fused step needs distinct src and dst buffers");
+    PLBM_TRY(require_params(h));
+    return launch_step(h, model, h->f[src], h->f[dst]);
+}
+
+int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) {
+    PLBM_REQUIRE(h && iold && inew, PLBM_ERR_ARG, "null argument");
+    PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
+    PLBM_TRY(check_buf(h, *iold));
+    PLBM_TRY(check_buf(h, *inew));
+    PLBM_REQUIRE(*iold != *inew, PLBM_ERR_ARG, "iold == inew");
+    PLBM_TRY(require_params(h));
+    for (int s = 0; s < nsteps; ++s) {
+        PLBM_TRY(launch_step(h, model, h->f[*iold], h->f[*inew]));
+        std::swap(*iold, *inew);
+    }
+    return PLBM_OK;
+}
+
+int plbm_stream(plbm_lattice *h, int src, int dst) {
+    PLBM_TRY(check_buf(h, src));
+    PLBM_TRY(check_buf(h, dst));
+    PLBM_REQUIRE(src != dst, PLBM_ERR_ARG, "streaming needs distinct src and dst buffers");
+    return dispatch_storage(h->storage, [&](auto S) {
+        using T = typename Traits<decltype(S)::value>::T;
+        hipLaunchKernelGGL(k_stream<T>, cell_grid(h), dim3(256), 0, h->stream,
+                           static_cast<const T *>(h->f[src]), static_cast<T *>(h->f[dst]), h->nx,
+                           h->ny, h->ldd, h->plane);
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    });
+}
+
+int plbm_collide(plbm_lattice *h, int model, int buf) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_TRY(require_params(h));
+    StepArgs a = make_args(h, h->f[buf], h->f[buf]);
+    return dispatch_model(model, [&](auto M) {
+        return dispatch_storage(h->storage, [&](auto S) {
+            hipLaunchKernelGGL((k_collide<decltype(M)::value, decltype(S)::value>), cell_grid(h),
+                               dim3(256), 0, h->stream, a);
+            PLBM_HIP(hipGetLastError());
+            return PLBM_OK;
+        });
+    });
+}
+
+int plbm_copy(plbm_lattice *h, int src, int dst) {
+    PLBM_TRY(check_buf(h, src));
+    PLBM_TRY(check_buf(h, dst));
+    if (src == dst) return PLBM_OK;
+    PLBM_HIP(hipMemcpyAsync(h->f[dst], h->f[src], 9 * h->plane * storage_bytes(h->storage),
+                            hipMemcpyDeviceToDevice, h->stream));
+    return PLBM_OK;
+}
+
+int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
+    return dispatch_storage(h->storage, [&](auto S) {
+        hipLaunchKernelGGL(k_macros<decltype(S)::value>, cell_grid(h), dim3(256), 0, h->stream,
+                           h->f[buf], h->nx, h->ny, h->ldd, h->plane, float(1.0 / h->scale), rho,
+                           ux, uy);
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    });
+}
+
+int plbm_macros(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
+    const size_t n = (size_t)h->nx * h->ny;
+    PLBM_TRY(ensure_tmp(h, 3 * n * sizeof(double)));
+    PLBM_TRY(plbm_macros_dev(h, buf, h->d_tmp, h->d_tmp + n, h->d_tmp + 2 * n));
+    PLBM_HIP(hipMemcpyAsync(rho, h->d_tmp, n * 8, hipMemcpyDeviceToHost, h->stream));
+    PLBM_HIP(hipMemcpyAsync(ux, h->d_tmp + n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    PLBM_HIP(hipMemcpyAsync(uy, h->d_tmp + 2 * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+    PLBM_HIP(hipStreamSynchronize(h->stream));
+    return PLBM_OK;
+}
+
+int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                    double out[PLBM_NRED]) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output");
+    PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
+                 "uxa and uya must both be given or both be NULL");
+    if (!h->d_partials) PLBM_HIP(hipMalloc(&h->d_partials, kReduceBlocks * PLBM_NRED * 8));
+    if (!h->d_red) PLBM_HIP(hipMalloc(&h->d_red, PLBM_NRED * 8));
+    const size_t n = (size_t)h->nx * h->ny;
+    const int nb = (int)std::min<size_t>(kReduceBlocks, (n + 255) / 256);
+    PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
+        hipLaunchKernelGGL(k_reduce<decltype(S)::value>, dim3(nb), dim3(256), 0, h->stream,
+                           h->f[buf], h->nx, h->ny, h->ldd, h->plane, float(1.0 / h->scale), uxa,
+                           uya, h->d_partials);
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    }));
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(PLBM_NRED), 0, h->stream, h->d_partials, nb,
+                       h->d_red);
+    PLBM_HIP(hipGetLastError());
+    PLBM_HIP(hipMemcpyAsync(out, h->d_red, PLBM_NRED * 8, hipMemcpyDeviceToHost, h->stream));
+    PLBM_HIP(hipStreamSynchronize(h->stream));
+    return PLBM_OK;
+}
+
+int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                double out[PLBM_NRED]) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
+                 "uxa and uya must both be given or both be NULL");
+    if (!uxa) return plbm_reduce_dev(h, buf, nullptr, nullptr, out);
+    const size_t n = (size_t)h->nx * h->ny;
+    PLBM_TRY(ensure_tmp(h, 2 * n * sizeof(double)));
+    PLBM_HIP(hipMemcpyAsync(h->d_tmp, uxa, n * 8, hipMemcpyHostToDevice, h->stream));
+    PLBM_HIP(hipMemcpyAsync(h->d_tmp + n, uya, n * 8, hipMemcpyHostToDevice, h->stream));
+    return plbm_reduce_dev(h, buf, h->d_tmp, h->d_tmp + n, out);
+}
+
+int plbm_sync(plbm_lattice *h) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_HIP(hipStreamSynchronize(h->stream));
+    return PLBM_OK;
+}
+
+int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
+    PLBM_REQUIRE(ms && iters > 0 && bytes >= 16, PLBM_ERR_ARG, "bad argument");
+    const size_t n = bytes / 16;
+    v4f *a = nullptr, *b = nullptr;
+    PLBM_HIP(hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        set_error("hipMalloc failed in copy probe");
+        return PLBM_ERR_NOMEM;
+    }
+    hipEvent_t e0, e1;
+    PLBM_HIP(hipEventCreate(&e0));
+    PLBM_HIP(hipEventCreate(&e1));
+    PLBM_HIP(hipMemset(a, 0, n * 16));
+    int dev = 0, ncu = 0;
+    PLBM_HIP(hipGetDevice(&dev));
+    PLBM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned grid = (unsigned)std::max(1, ncu) * 32;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, a, b, n);  // warm-up
+    PLBM_HIP(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, a, b, n);
+    PLBM_HIP(hipEventRecord(e1, nullptr));
+    PLBM_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PLBM_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return PLBM_OK;
+}
+
+}  // extern "C"

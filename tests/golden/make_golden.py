@@ -1,0 +1,150 @@
+"""Write the golden fixtures from the reference itself.
+
+Runs the reference Fortran hot path (oracle/_ref/libplbm_ref.so, flang-built from the
+ivan-pi/periodic-lbm sources by oracle/Makefile) and stores inputs and outputs as small
+data files in this directory:
+
+  ref_small.npz   full populations of tiny grids (8x8, 5x3, 16x16, 12x20) after 1, 10
+                  and 100 steps of BGK / TRT / RR, plus update_macros of the lagged buffer
+  ref_config1.json  the 64x64 Taylor-Green case (ν=0.01, u0=0.05, 1000 steps, Λ=1/4) for
+                  BGK / TRT / RR: sha256 of the final two population buffers (bitwise
+                  fingerprint), the relative L2 errors vs the analytic solution, mass
+  ref_params.json set_properties / lambda_d / magic_number / equilibrium values
+  ref_app.json    the unchanged app's printed results (main_taylor_green 1.0)
+
+Usage (in a container that has the reference tree and flang):
+  make -C oracle ref && python tests/golden/make_golden.py [--app]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT / "oracle"))
+import pyoracle as po  # noqa: E402
+
+MODELS = ["bgk", "trt", "rr"]
+
+
+def tgv_init(R, nx, ny, u0, nu):
+    k = 2 * np.pi / nx
+    p, ux, uy = R.tgv(nx, ny, k, k, u0, nu, 0.0)
+    return p / (1.0 / 3.0) + 1.0, ux, uy
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(np.asfortranarray(a)).tobytes(order="F")).hexdigest()
+
+
+def small(R):
+    out = {}
+    cases = [(8, 8, 0.05, 0.01), (5, 3, 0.03, 0.02), (16, 16, 0.05, 0.01), (12, 20, 0.04, 0.005)]
+    for nx, ny, u0, nu in cases:
+        # non-square grids: the reference's TGV eval only covers square grids
+        # (src/benchmarks/taylor_green.f90:68), so the field is a seeded random one
+        if nx == ny:
+            rho, ux, uy = tgv_init(R, nx, ny, u0, nu)
+        else:
+            rng = np.random.default_rng(nx * 100 + ny)
+            rho = np.asfortranarray(1.0 + 0.01 * rng.standard_normal((ny, nx)))
+            ux = np.asfortranarray(u0 * rng.standard_normal((ny, nx)))
+            uy = np.asfortranarray(u0 * rng.standard_normal((ny, nx)))
+        key = f"{nx}x{ny}"
+        out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+        out[f"{key}_params"] = np.array([nu, 1.0, 0.25])
+        for m in MODELS:
+            g = R.grid(nx, ny, m)
+            g.set_properties(nu, 1.0, 0.25)
+            g.set_macros(rho, ux, uy)
+            g.set_pdf_to_equilibrium()
+            done = 0
+            for n in (1, 10, 100):
+                g.step(n - done)
+                done = n
+                iold, inew, _, _ = g.indices()
+                out[f"{key}_{m}_{n}_latest"] = g.get_f(iold)[:ny]
+                g.update_macros()
+                r, u, v = g.macros()
+                out[f"{key}_{m}_{n}_macros"] = np.stack([r, u, v], axis=2)
+            g.close()
+    np.savez_compressed(HERE / "ref_small.npz", **out)
+
+
+def config1(R):
+    n, u0, nu, steps = 64, 0.05, 0.01, 1000
+    k = 2 * np.pi / n
+    res = {"n": n, "u0": u0, "nu": nu, "steps": steps, "magic": 0.25, "dt": 1.0, "models": {}}
+    rho, ux, uy = tgv_init(R, n, n, u0, nu)
+    _, uxa, uya = R.tgv(n, n, k, k, u0, nu, float(steps))
+    for m in MODELS:
+        g = R.grid(n, n, m)
+        g.set_properties(nu, 1.0, 0.25)
+        g.set_macros(rho, ux, uy)
+        g.set_pdf_to_equilibrium()
+        mass0 = float(np.sum(rho))
+        g.step(steps)
+        iold, inew, _, _ = g.indices()
+        latest, prev = g.get_f(iold)[:n], g.get_f(inew)[:n]
+        g.update_macros()
+        r, u, v = g.macros()
+        l2lag = float(np.sqrt(np.sum((u - uxa) ** 2 + (v - uya) ** 2) / np.sum(uxa ** 2 + uya ** 2)))
+        res["models"][m] = {
+            "sha256_latest": sha(latest), "sha256_prev": sha(prev),
+            "sha256_macros_prev": sha(np.stack([r, u, v], axis=2)),
+            "l2_lag": l2lag, "mass0": mass0, "mass_prev": float(np.sum(r)),
+            "omega": g.params()["omega"],
+            "sample_latest": latest[::8, ::8, :].ravel(order="F").tolist(),
+        }
+        g.close()
+    (HERE / "ref_config1.json").write_text(json.dumps(res, indent=1))
+
+
+def params(R):
+    g = R.grid(4, 4, "trt")
+    out = {"set_properties": [], "lambda_d": [], "magic_number": [], "equilibrium": []}
+    for nu, dt, magic in [(0.01, 1.0, 0.25), (0.005, 1.0, None), (0.0554, 1.0, 0.25),
+                          (0.1, 0.5, None), (1e-3, 2.0, 3 / 16)]:
+        g.set_properties(nu, dt, magic)
+        out["set_properties"].append({"nu": nu, "dt": dt, "magic": magic, **g.params()})
+    for om, x in [(1.9, 0.25), (1.0, 1 / 12), (0.5, 0.25), (1.99, 3 / 16), (1.2, 1.0)]:
+        out["lambda_d"].append([om, x, R.lambda_d(om, x)])
+    for le, ld in [(1.9, 0.1), (1.0, 1.0), (0.6, 1.7)]:
+        out["magic_number"].append([le, ld, R.magic_number(le, ld)])
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        rho, ux, uy = 1 + 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal()
+        out["equilibrium"].append([rho, ux, uy, R.equilibrium(rho, ux, uy).tolist()])
+    g.close()
+    (HERE / "ref_params.json").write_text(json.dumps(out, indent=1))
+
+
+def app():
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([str(po.REF_APP), "1.0"], cwd=d, capture_output=True, text=True,
+                           check=True)
+    vals = {}
+    for line in r.stdout.splitlines():
+        s = line.strip()
+        for key in ("L2-norm =", "Final time =", "nsteps =", "omega =", "tc   ="):
+            if s.startswith(key):
+                vals[key.rstrip(" =").strip()] = float(s.split("=")[1])
+    (HERE / "ref_app.json").write_text(json.dumps({"args": ["1.0"], "stdout": r.stdout,
+                                                   "values": vals}, indent=1))
+
+
+if __name__ == "__main__":
+    R = po.RefLib()
+    params(R)
+    small(R)
+    config1(R)
+    if "--app" in sys.argv:
+        app()
+    print("golden fixtures written to", HERE)

@@ -1,0 +1,86 @@
+"""The oracle (C restatement) against the golden fixtures written by the reference
+Fortran itself (tests/golden/make_golden.py).  Everything here is bit-exact."""
+from __future__ import annotations
+
+import hashlib
+import json
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+import pyoracle as po
+
+MODELS = ["bgk", "trt", "rr"]
+SMALL = [(8, 8), (5, 3), (16, 16), (12, 20)]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(np.asfortranarray(a)).tobytes(order="F")).hexdigest()
+
+
+@pytest.mark.parametrize("nx,ny", SMALL)
+@pytest.mark.parametrize("model", MODELS)
+def test_small_grids_bitwise(oracle, golden_small, nx, ny, model):
+    G = golden_small
+    key = f"{nx}x{ny}"
+    rho, ux, uy = (np.asfortranarray(G[f"{key}_{c}"]) for c in ("rho", "ux", "uy"))
+    nu, dt, magic = G[f"{key}_params"]
+    par = oracle.set_properties(nu, dt, magic)
+    fa, fb = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, fa)
+    a, b, done = fa, fb, 0
+    for n in (1, 10, 100):
+        latest, prev = oracle.run(model, nx, ny, a, b, n - done, par["omega"], par["trt_magic"])
+        done = n
+        a, b = latest, prev
+        np.testing.assert_array_equal(latest[:ny], G[f"{key}_{model}_{n}_latest"])
+        r, u, v = oracle.update_macros(nx, ny, prev)
+        np.testing.assert_array_equal(np.stack([r, u, v], axis=2), G[f"{key}_{model}_{n}_macros"])
+        # continue from the latest state: swap so that the next run starts from it
+        a, b = latest, prev
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_config1_taylor_green_bitwise(oracle, model):
+    """64x64 TGV, u0=0.05, nu=0.01, 1000 steps: the reference's final buffers, bit for bit."""
+    C = json.loads((GOLDEN / "ref_config1.json").read_text())
+    n, u0, nu, steps = C["n"], C["u0"], C["nu"], C["steps"]
+    M = C["models"][model]
+    k = 2 * np.pi / n
+    p, ux, uy = oracle.tgv(n, n, k, k, u0, nu, 0.0)
+    rho = np.asfortranarray(p / (1.0 / 3.0) + 1.0)
+    par = oracle.set_properties(nu, 1.0, 0.25)
+    assert par["omega"] == M["omega"]
+    fa, fb = po.new_f(n, n), po.new_f(n, n)
+    oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, fa)
+    latest, prev = oracle.run(model, n, n, fa, fb, steps, par["omega"], par["trt_magic"])
+    assert sha(latest[:n]) == M["sha256_latest"]
+    assert sha(prev[:n]) == M["sha256_prev"]
+    r, u, v = oracle.update_macros(n, n, prev)
+    assert sha(np.stack([r, u, v], axis=2)) == M["sha256_macros_prev"]
+    _, uxa, uya = oracle.tgv(n, n, k, k, u0, nu, float(steps))
+    l2 = np.sqrt(np.sum((u - uxa) ** 2 + (v - uya) ** 2) / np.sum(uxa ** 2 + uya ** 2))
+    assert l2 == M["l2_lag"]
+
+
+def test_params_and_equilibrium(oracle):
+    G = json.loads((GOLDEN / "ref_params.json").read_text())
+    for rec in G["set_properties"]:
+        par = oracle.set_properties(rec["nu"], rec["dt"], rec["magic"])
+        for key in ("tau", "omega", "trt_magic", "csqr"):
+            assert par[key] == rec[key], (rec, key)
+    for om, x, v in G["lambda_d"]:
+        assert oracle.lambda_d(om, x) == v
+    for le, ld, v in G["magic_number"]:
+        assert oracle.magic_number(le, ld) == v
+    for rho, ux, uy, feq in G["equilibrium"]:
+        np.testing.assert_array_equal(oracle.equilibrium(rho, ux, uy), np.array(feq))
+
+
+def test_reference_app_record():
+    """The unchanged app's printed L2 (app/main_taylor_green.f90:125-126) as recorded."""
+    A = json.loads((GOLDEN / "ref_app.json").read_text())
+    assert A["values"]["L2-norm"] == pytest.approx(3.7266899791616004e-05, rel=0, abs=0)
+    assert A["values"]["Final time"] == 48658.0

@@ -1,0 +1,280 @@
+"""Pins of the oracle that do not depend on the reference: closed forms, invariants,
+special cases that reduce to textbook formulas, brute force on tiny inputs.  Each is
+chosen so that a plausible slip in the restatement (a dropped term, a wrong sign, a
+swapped index or direction) fails at least one of them."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+MODELS = ["bgk", "trt", "rr"]
+
+
+def moments(f):
+    """f: (..., 9) -> rho, jx, jy, Pxx, Pyy, Pxy"""
+    return (f.sum(-1), (f * CX).sum(-1), (f * CY).sum(-1), (f * CX * CX).sum(-1),
+            (f * CY * CY).sum(-1), (f * CX * CY).sum(-1))
+
+
+def random_state(nx, ny, seed, amp=0.05, ld=None):
+    rng = np.random.default_rng(seed)
+    rho = 1 + 0.02 * rng.standard_normal((ny, nx))
+    ux, uy = amp * rng.standard_normal((ny, nx)), amp * rng.standard_normal((ny, nx))
+    f = po.new_f(nx, ny, ld)
+    feq = np.stack([W[q] * rho * (1 + 3 * (CX[q] * ux + CY[q] * uy) + 4.5 * (CX[q] * ux + CY[q] * uy) ** 2
+                                  - 1.5 * (ux ** 2 + uy ** 2)) for q in range(9)], axis=2)
+    f[:ny] = feq * (1 + 0.01 * rng.standard_normal(feq.shape))  # off-equilibrium
+    return f
+
+
+def test_equilibrium_moments(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        rho, ux, uy = 1 + 0.2 * rng.random(), 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal()
+        r, jx, jy, pxx, pyy, pxy = moments(oracle.equilibrium(rho, ux, uy))
+        assert r == pytest.approx(rho, rel=1e-15)
+        assert jx == pytest.approx(rho * ux, abs=1e-15)
+        assert jy == pytest.approx(rho * uy, abs=1e-15)
+        assert pxx == pytest.approx(rho * (ux * ux + 1 / 3), rel=1e-14)
+        assert pyy == pytest.approx(rho * (uy * uy + 1 / 3), rel=1e-14)
+        assert pxy == pytest.approx(rho * ux * uy, abs=1e-15)
+    np.testing.assert_allclose(oracle.equilibrium(1.3, 0.0, 0.0), 1.3 * W, rtol=1e-15)
+
+
+def test_stream_is_the_periodic_shift(oracle):
+    """Brute force: f_q(y, x) = f_q(y - cy_q, x - cx_q) with periodic wrap (np.roll)."""
+    for nx, ny in [(4, 4), (5, 3), (1, 7), (7, 1), (3, 16)]:
+        ld = po.padded_ld(ny)
+        src = po.new_f(nx, ny, ld)
+        src[:ny] = np.arange(ny * nx * 9, dtype=float).reshape(ny, nx, 9, order="F") + 1
+        dst = po.new_f(nx, ny, ld)
+        oracle.stream(nx, ny, src, dst)
+        for q in range(9):
+            want = np.roll(src[:ny, :, q], shift=(CY[q], CX[q]), axis=(0, 1))
+            np.testing.assert_array_equal(dst[:ny, :, q], want)
+        assert dst[:ny].sum() == src[:ny].sum()
+        assert not dst[ny:].any()  # padding rows untouched
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_collisions_conserve_mass_and_momentum(oracle, model):
+    nx, ny = 9, 7
+    f = random_state(nx, ny, 1)
+    before = moments(f[:ny])
+    oracle.collide(model, nx, ny, f, 1.7, 0.25)
+    after = moments(f[:ny])
+    np.testing.assert_allclose(after[0], before[0], rtol=2e-15)
+    if model == "trt":  # incompressible: j = sum c f is conserved directly
+        np.testing.assert_allclose(after[1], before[1], atol=2e-16)
+        np.testing.assert_allclose(after[2], before[2], atol=2e-16)
+    else:
+        np.testing.assert_allclose(after[1], before[1], atol=2e-16)
+        np.testing.assert_allclose(after[2], before[2], atol=2e-16)
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_rest_state_is_fixed_point(oracle, model):
+    nx, ny = 4, 5
+    f = po.new_f(nx, ny)
+    f[:ny] = 1.07 * W
+    g = f.copy(order="F")
+    oracle.collide(model, nx, ny, g, 1.3, 0.25)
+    np.testing.assert_allclose(g[:ny], f[:ny], rtol=1e-15)
+
+
+def test_bgk_omega_one_gives_equilibrium(oracle):
+    nx, ny = 6, 5
+    f = random_state(nx, ny, 2)
+    rho, jx, jy = moments(f[:ny])[:3]
+    oracle.collide("bgk", nx, ny, f, 1.0)
+    for y in range(ny):
+        for x in range(nx):
+            want = oracle.equilibrium(rho[y, x], jx[y, x] / rho[y, x], jy[y, x] / rho[y, x])
+            np.testing.assert_allclose(f[y, x], want, rtol=1e-14)
+
+
+def test_bgk_relaxation_formula(oracle):
+    """f' = f + omega (feq - f) for general omega (textbook BGK)."""
+    nx, ny, om = 5, 4, 1.63
+    f = random_state(nx, ny, 3)
+    f0 = f.copy(order="F")
+    oracle.collide("bgk", nx, ny, f, om)
+    rho, jx, jy = moments(f0[:ny])[:3]
+    for y in range(ny):
+        for x in range(nx):
+            feq = oracle.equilibrium(rho[y, x], jx[y, x] / rho[y, x], jy[y, x] / rho[y, x])
+            np.testing.assert_allclose(f[y, x], f0[y, x] + om * (feq - f0[y, x]), rtol=1e-13)
+
+
+def incompressible_feq(rho, jx, jy):
+    cu = CX * jx + CY * jy
+    return W * (rho + 3 * cu + 4.5 * cu * cu - 1.5 * (jx * jx + jy * jy))
+
+
+def test_trt_equal_rates_is_incompressible_bgk(oracle):
+    """lambda_odd == lambda_even reduces TRT to BGK towards the incompressible feq."""
+    nx, ny, le = 5, 6, 1.45
+    f = random_state(nx, ny, 4)
+    f0 = f.copy(order="F")
+    oracle.collide_trt(nx, ny, f, le, le)
+    rho, jx, jy = moments(f0[:ny])[:3]
+    for y in range(ny):
+        for x in range(nx):
+            feq = incompressible_feq(rho[y, x], jx[y, x], jy[y, x])
+            np.testing.assert_allclose(f[y, x], f0[y, x] + le * (feq - f0[y, x]), rtol=1e-13,
+                                       atol=1e-16)
+
+
+def test_trt_odd_part_relaxes_with_lambda_odd(oracle):
+    """Antisymmetric parts relax with lambda_odd, symmetric ones with lambda_even."""
+    nx, ny, le, lo = 4, 4, 1.3, 0.55
+    f = random_state(nx, ny, 5)
+    f0 = f.copy(order="F")
+    oracle.collide_trt(nx, ny, f, le, lo)
+    rho, jx, jy = moments(f0[:ny])[:3]
+    opp = [0, 3, 4, 1, 2, 7, 8, 5, 6]
+    for y in range(ny):
+        for x in range(nx):
+            feq = incompressible_feq(rho[y, x], jx[y, x], jy[y, x])
+            a, e = f0[y, x], feq
+            sym = 0.5 * (a + a[opp]) - le * 0.5 * ((a + a[opp]) - (e + e[opp]))
+            asym = 0.5 * (a - a[opp]) - lo * 0.5 * ((a - a[opp]) - (e - e[opp]))
+            np.testing.assert_allclose(f[y, x], sym + asym, rtol=1e-13, atol=1e-16)
+
+
+def test_magic_parameter_identities(oracle):
+    for le in (0.5, 1.0, 1.5, 1.94):
+        for L in (1 / 12, 3 / 16, 0.25, 1.0):
+            lo = oracle.lambda_d(le, L)
+            assert oracle.magic_number(le, lo) == pytest.approx(L, rel=1e-14)
+        assert oracle.lambda_d(le, 0.25) == pytest.approx(2 - le, rel=1e-15)
+
+
+def hermite4_feq(rho, ux, uy):
+    """4th-order Hermite equilibrium on D2Q9 (textbook form, w_q rho sum_n H_n:u^n/n!cs^2n)."""
+    cs2 = 1 / 3
+    out = np.zeros(9)
+    for q in range(9):
+        c = np.array([CX[q], CY[q]])
+        u = np.array([ux, uy])
+        H2 = np.outer(c, c) - cs2 * np.eye(2)
+        t1 = c @ u / cs2
+        t2 = np.sum(H2 * np.outer(u, u)) / (2 * cs2 ** 2)
+        hxxy = H2[0, 0] * c[1]
+        hxyy = H2[1, 1] * c[0]
+        t3 = (hxxy * ux * ux * uy + hxyy * ux * uy * uy) * 3 / (6 * cs2 ** 3)
+        t4 = H2[0, 0] * H2[1, 1] * ux * ux * uy * uy * 6 / (24 * cs2 ** 4)
+        out[q] = W[q] * rho * (1 + t1 + t2 + t3 + t4)
+    return out
+
+
+def test_rr_omega_one_gives_hermite4_equilibrium(oracle):
+    nx, ny = 4, 3
+    f = random_state(nx, ny, 6)
+    rho, jx, jy = moments(f[:ny])[:3]
+    oracle.collide("rr", nx, ny, f, 1.0)
+    for y in range(ny):
+        for x in range(nx):
+            want = hermite4_feq(rho[y, x], jx[y, x] / rho[y, x], jy[y, x] / rho[y, x])
+            np.testing.assert_allclose(f[y, x], want, rtol=1e-13)
+
+
+def test_update_macros_definition(oracle):
+    nx, ny = 7, 5
+    f = random_state(nx, ny, 7, ld=16)
+    r, u, v = oracle.update_macros(nx, ny, f)
+    rho, jx, jy = moments(f[:ny])[:3]
+    np.testing.assert_allclose(r, rho, rtol=1e-15)
+    np.testing.assert_allclose(u, jx / rho, rtol=1e-13, atol=1e-17)
+    np.testing.assert_allclose(v, jy / rho, rtol=1e-13, atol=1e-17)
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_shifted_forms_equal_unshifted_in_fp64(oracle, model):
+    """Deviation storage g = f - w rho0 is the same collision (FP64, to roundoff)."""
+    nx, ny, om = 6, 6, 1.8
+    f = random_state(nx, ny, 8)
+    g = f.copy(order="F")
+    g[:ny] -= W
+    lo = oracle.lambda_d(om, 0.25)
+    oracle.collide(model, nx, ny, f, om, 0.25)
+    oracle.collide_shifted(model, nx, ny, g, om, lo)
+    np.testing.assert_allclose(g[:ny], f[:ny] - W, rtol=0, atol=3e-16)
+    rs, us, vs = oracle.update_macros(nx, ny, g, shifted=True)
+    r, u, v = oracle.update_macros(nx, ny, f)
+    np.testing.assert_allclose(rs, r, rtol=2e-15)
+    np.testing.assert_allclose(us, u, atol=1e-16)
+
+
+def test_shifted_equilibrium(oracle):
+    for rho, ux, uy in [(1.02, 0.03, -0.04), (0.97, -0.1, 0.05)]:
+        np.testing.assert_allclose(oracle.equilibrium(rho, ux, uy, shifted=True) + W,
+                                   oracle.equilibrium(rho, ux, uy), atol=2e-16)
+
+
+def test_taylor_green_closed_form(oracle):
+    """The textbook TGV: divergence-free, decays as exp(-nu (kx^2+ky^2) t)."""
+    n, u0, nu = 32, 0.05, 0.01
+    k = 2 * np.pi / n
+    p0, ux0, uy0 = oracle.tgv(n, n, k, k, u0, nu, 0.0)
+    p1, ux1, uy1 = oracle.tgv(n, n, k, k, u0, nu, 500.0)
+    decay = np.exp(-nu * 2 * k * k * 500.0)
+    np.testing.assert_allclose(ux1, ux0 * decay, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(p1, p0 * decay ** 2, rtol=1e-12, atol=1e-18)
+    x = np.arange(n) + 0.5
+    X, Y = np.meshgrid(x, x)  # (ny, nx)
+    np.testing.assert_allclose(ux0, -u0 * np.cos(k * X) * np.sin(k * Y), atol=1e-16)
+    np.testing.assert_allclose(uy0, u0 * np.sin(k * X) * np.cos(k * Y), atol=1e-16)
+
+
+def test_lbm_decay_matches_analytic(oracle):
+    """Discrete TGV decay agrees with exp(-2 nu k^2 t) to O(k^2, Ma^2)."""
+    n, u0, nu, steps = 32, 0.02, 0.02, 400
+    k = 2 * np.pi / n
+    p, ux, uy = oracle.tgv(n, n, k, k, u0, nu, 0.0)
+    rho = np.asfortranarray(p * 3 + 1)
+    par = oracle.set_properties(nu, 1.0, 0.25)
+    for model in MODELS:
+        fa, fb = po.new_f(n, n), po.new_f(n, n)
+        oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, fa)
+        latest, _ = oracle.run(model, n, n, fa, fb, steps, par["omega"], 0.25)
+        _, u, v = oracle.update_macros(n, n, latest)
+        _, ua, va = oracle.tgv(n, n, k, k, u0, nu, float(steps))
+        assert oracle.l2_norm(u, v, ua, va) < 1e-2, model
+
+
+def test_noise_generator_restatement(oracle):
+    """splitmix64 output function over seed + (i+1)*gamma, top 53 bits (numpy restatement)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        for seed in (0, 1, 7, 2 ** 40 + 3):
+            i = np.arange(100, dtype=np.uint64)
+            z = np.uint64(seed) + (i + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            want = (z >> np.uint64(11)).astype(np.float64) / 2.0 ** 53
+            got = np.array([oracle.noise(seed, int(c)) for c in range(100)])
+            np.testing.assert_array_equal(got, want)
+
+
+def test_rr_nonequilibrium_stress_relaxes_by_one_minus_omega(oracle):
+    """Regularized collision: the second-order non-equilibrium moments of the output are
+    (1 - omega) times those of the input, and the result is the Hermite-4 equilibrium
+    plus a purely regularized (Hermite-projected) non-equilibrium part."""
+    nx, ny, om = 5, 4, 1.37
+    f = random_state(nx, ny, 9)
+    f0 = f.copy(order="F")
+    oracle.collide("rr", nx, ny, f, om)
+    rho, jx, jy = moments(f0[:ny])[:3]
+    for y in range(ny):
+        for x in range(nx):
+            feq = hermite4_feq(rho[y, x], jx[y, x] / rho[y, x], jy[y, x] / rho[y, x])
+            pre = moments(f0[y, x] - feq)[3:]
+            post = moments(f[y, x] - feq)[3:]
+            np.testing.assert_allclose(post, (1 - om) * np.array(pre), rtol=1e-10, atol=1e-17)
