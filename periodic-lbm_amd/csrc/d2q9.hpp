@@ -195,6 +195,7 @@ __device__ inline void macros_ref(const double f[9], double &rho, double &ux, do
 // feq_q = w_q*rho*(1 + P_q)  =>  geq_q = feq_q - w_q*rho0 = w_q*(drho + rho*P_q).
 template <class R>
 __device__ inline void equilibrium_shifted(R drho, R ux, R uy, R geq[9]) {
+#pragma clang fp contract(fast)
     const R rho = R(1) + drho;
     const R uxx = ux * ux, uyy = uy * uy;
     const R p0 = R(-1.5) * (uxx + uyy);
@@ -214,6 +215,7 @@ __device__ inline void equilibrium_shifted(R drho, R ux, R uy, R geq[9]) {
 
 template <class R>
 __device__ inline void moments_shifted(const R g[9], R &drho, R &jx, R &jy) {
+#pragma clang fp contract(fast)
     drho = (((g[5] + g[7]) + (g[6] + g[8])) + ((g[1] + g[3]) + (g[2] + g[4]))) + g[0];
     jx = ((g[5] - g[7]) + (g[8] - g[6])) + (g[1] - g[3]);
     jy = ((g[5] - g[7]) + (g[6] - g[8])) + (g[2] - g[4]);
@@ -221,6 +223,7 @@ __device__ inline void moments_shifted(const R g[9], R &drho, R &jx, R &jy) {
 
 template <class R>
 __device__ inline void collide_bgk_shifted(R g[9], R omega) {
+#pragma clang fp contract(fast)
     R drho, jx, jy;
     moments_shifted(g, drho, jx, jy);
     const R inv = R(1) / (R(1) + drho);
@@ -234,6 +237,7 @@ __device__ inline void collide_bgk_shifted(R g[9], R omega) {
 // shift of f_q + f_qbar, and the rest population loses t0*rho0 on both sides.
 template <class R>
 __device__ inline void collide_trt_shifted(R g[9], R le, R lo) {
+#pragma clang fp contract(fast)
     R drho, jx, jy;
     moments_shifted(g, drho, jx, jy);
     const R jx2 = jx * jx, jy2 = jy * jy;
@@ -263,6 +267,7 @@ __device__ inline void collide_trt_shifted(R g[9], R le, R lo) {
 // 3rd/4th-order reconstruction unchanged.
 template <class R>
 __device__ inline void collide_rr_shifted(R g[9], R omega) {
+#pragma clang fp contract(fast)
     R drho, jx, jy;
     moments_shifted(g, drho, jx, jy);
     const R rho = R(1) + drho;
@@ -306,6 +311,35 @@ __device__ inline void collide_rr_shifted(R g[9], R omega) {
     g[7] = geq[7] + R(WD) * om * (id - R(9) * (axxy + ayyx - axy));
     g[6] = geq[6] + R(WD) * om * (id + R(9) * (axxy - ayyx - axy));
     g[8] = geq[8] + R(WD) * om * (id - R(9) * (axxy - ayyx + axy));
+}
+
+// Conservation restoration for reduced-precision arithmetic.  In FP32 the weights
+// w_q (4/9, 1/9, 1/36) and the TRT/RR prefactors are not representable, so the
+// equilibrium's zeroth and first moments miss rho and j by a fixed relative amount
+// (~1e-8) in every cell and step; over thousands of steps that bias accumulates in the
+// velocity field.  Subtracting w_q*(dd + 3 c_q.dj) from the post-collision populations
+// (dd, dj = moment change the collision produced) returns mass and momentum to their
+// pre-collision values up to unbiased rounding.  In exact arithmetic the correction is
+// zero for every model (each conserves mass and momentum), so it changes no result
+// beyond rounding.
+template <class R>
+__device__ inline void restore_conservation(R g[9], R d0, R jx0, R jy0) {
+#pragma clang fp contract(fast)
+    R d1, jx1, jy1;
+    moments_shifted(g, d1, jx1, jy1);
+    const R dd = d1 - d0, dx = jx1 - jx0, dy = jy1 - jy0;
+    const R a0 = R(W0) * dd, as = R(WS) * dd, ad = R(WD) * dd;
+    const R bsx = R(3.0 * WS) * dx, bsy = R(3.0 * WS) * dy;
+    const R bdp = R(3.0 * WD) * (dx + dy), bdm = R(3.0 * WD) * (dx - dy);
+    g[0] -= a0;
+    g[1] -= as + bsx;
+    g[3] -= as - bsx;
+    g[2] -= as + bsy;
+    g[4] -= as - bsy;
+    g[5] -= ad + bdp;
+    g[7] -= ad - bdp;
+    g[8] -= ad + bdm;
+    g[6] -= ad - bdm;
 }
 
 }  // namespace plbm
