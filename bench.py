@@ -71,7 +71,11 @@ def gpu_leg(args):
     P = g.load_package()
     torch.cuda.set_device(0)
     dev_name = torch.cuda.get_device_name(0)
-    stream = torch.cuda.current_stream().cuda_stream
+    # one explicit stream for the engine and the timing events (torch's default stream
+    # would be passed as NULL, which the engine reads as "use the lattice's own stream")
+    ts = torch.cuda.Stream()
+    torch.cuda.set_stream(ts)
+    stream = ts.cuda_stream
     n = args.n
     cells = n * n
     rho, ux, uy = initial_fields(P, n)
@@ -85,10 +89,13 @@ def gpu_leg(args):
         lat.set_params(omega, MAGIC)
         lat.equilibrium_init_dev(0, d_rho.data_ptr(), d_ux.data_ptr(), d_uy.data_ptr())
         torch.cuda.synchronize()
+        lat.autotune("trt", 0, reps=5)
+        block, remap = lat.get_launch("trt")
         ms, _ = timed_run(P, torch, lat, "trt", args.steps, args.warmup)
         mlups = cells / (ms * 1e-3) / 1e6
         results[storage] = {"ms_per_step": ms, "mlups": mlups,
-                            "gbs": mlups * 1e6 * BYTES_PER_LU[storage] / 1e9}
+                            "gbs": mlups * 1e6 * BYTES_PER_LU[storage] / 1e9,
+                            "launch": {"block": block, "xcd_remap": remap}}
         lat.close()
         torch.cuda.empty_cache()
 
@@ -214,6 +221,11 @@ def main():
         "storage_forms": {
             k: {kk: (round(vv, 6) if isinstance(vv, float) and vv > 1e-3 else vv)
                 for kk, vv in v.items()} for k, v in results.items()},
+        "roofline_by_storage": {
+            k: {"bytes_per_lu": BYTES_PER_LU[k], "achieved_gbs": round(v["gbs"], 1),
+                "frac_of_peak": round(v["gbs"] / HBM_PEAK_GBS, 4),
+                "frac_of_measured_copy": round(v["gbs"] / copy_gbs, 4)}
+            for k, v in results.items()},
         "f32s_speedup_vs_f64": round(results["f32s"]["mlups"] / f64["mlups"], 3),
         "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
     }

@@ -87,7 +87,8 @@ int plbm_destroy(plbm_lattice *h);
 int plbm_get_info(const plbm_lattice *h, plbm_info *out);
 /* Run this lattice's work on `stream` (a hipStream_t; NULL = the lattice's own). */
 int plbm_set_stream(plbm_lattice *h, void *stream);
-/* Device pointer of population buffer `buf` (layout [q][x][ldd], storage type). */
+/* Device pointer of population buffer `buf`.  Device layout (column-interleaved SoA):
+ * element (q, x, y) at index (x*9 + q)*ldd + y, in the storage type. */
 int plbm_buffer_ptr(plbm_lattice *h, int buf, void **dptr);
 
 /* set_properties (src/fvm_bardow.F90:242-269) computes omega and trt_magic on the host
@@ -161,6 +162,18 @@ double plbm_noise(uint64_t seed, uint64_t counter);
 /* lambda_d / magic_number (src/collision_trt.F90:24-34). */
 double plbm_lambda_d(double omega, double x);
 double plbm_magic_number(double le, double ld);
+
+/* ---------------------------------------------------------------- launch shape */
+
+/* Launch shape of the fused step for one model: threads per workgroup (multiple of 64 in
+ * [64, 1024]) and XCD-contiguous workgroup renumbering (0/1).  Default 256 / 1. */
+int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap);
+int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap);
+/* Autotuner: time every launch shape of the fused step for `model` (reps launches from
+ * buffer buf into a scratch buffer; the lattice's buffers are not modified), keep the
+ * fastest for this lattice and cache it per (device, storage, model, grid) for the
+ * process.  *best_ms (may be NULL) receives the winner's time per step. */
+int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms);
 
 /* ---------------------------------------------------------------- measurement */
 

@@ -78,6 +78,9 @@ SIGNATURES = {
     "plbm_lambda_d": (_d, [_d, _d]),
     "plbm_magic_number": (_d, [_d, _d]),
     "plbm_copy_bandwidth": (_i, [_sz, _i, _dp]),
+    "plbm_set_launch": (_i, [_vp, _i, _i, _i]),
+    "plbm_get_launch": (_i, [_vp, _i, _ip, _ip]),
+    "plbm_autotune": (_i, [_vp, _i, _i, _i, _dp]),
 }
 
 _LIB = None

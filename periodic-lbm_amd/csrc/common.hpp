@@ -66,12 +66,30 @@ inline size_t storage_bytes(int st) {
     return st == PLBM_F16S ? 2 : (st == PLBM_F32S ? 4 : 8);
 }
 
+// Device layout of one population buffer: column-interleaved SoA.  Column x holds nine
+// contiguous q-segments of ldd elements (y unit-stride), so element (q, x, y) sits at
+// (x*9 + q)*ldd + y.  The nine read and nine write streams of a workgroup then lie
+// within a few hundred KB of each other instead of one plane (hundreds of MB, a power
+// of two at 8192^2) apart, which keeps HBM channels and banks evenly loaded.
+__host__ __device__ inline size_t pop_index(int q, int x, int y, int ldd) {
+    return ((size_t)x * 9 + (size_t)q) * (size_t)ldd + (size_t)y;
+}
+
+// Launch shape of the streaming kernels: threads per workgroup (a multiple of 64, at
+// most 1024) and whether workgroups are renumbered so each XCD owns a contiguous range
+// of tiles.
+struct LaunchCfg {
+    int block = 256;
+    int remap = 1;
+};
+
 // Per-launch scalars of one step, passed by value.
 struct StepArgs {
     const void *src;
     void *dst;
     int nx, ny, ldd, nby;   // nby = y-blocks per column
-    size_t plane;           // elements per q-plane = nx*ldd
+    size_t cs;              // column stride = 9*ldd elements
+    int remap;
     double omega, lambda_odd;
     float omega_f, lambda_odd_f;
     float scale, inv_scale; // FP16 storage scale (1 otherwise)
@@ -87,12 +105,12 @@ struct plbm_lattice {
     hipStream_t stream = nullptr;  // the stream work is issued on
     hipStream_t own = nullptr;     // the lattice's own stream
     void *f[plbm::kMaxBuffers] = {};
-    size_t plane = 0;              // elements per q-plane
+    size_t elems = 0;              // elements per buffer = 9*nx*ldd
     double omega = 0, trt_magic = 0, lambda_odd = 0;
     bool params_set = false;
     double scale = 1.0;            // FP16 storage scale
-    int block = 256;               // threads per block of the step kernel
-    bool xcd_remap = true;
+    plbm::LaunchCfg cfg[3];        // fused-step launch shape per collision model
+    bool tuned[3] = {false, false, false};
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

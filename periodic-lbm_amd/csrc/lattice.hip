@@ -1,9 +1,9 @@
 // lattice.hip — the single-lattice engine: device kernels and the plbm_* C ABI.
 //
-// HBM layout of one population buffer: nine q-planes, each nx columns of ldd elements,
-// y unit-stride — the reference's f(ld, nx, 0:8) (src/fvm_bardow.F90:153) with the
-// device pitch ldd = ny rounded up to 64 elements so every column starts 128-byte
-// aligned for FP16 and 512-byte aligned for FP64.
+// HBM layout of one population buffer: column-interleaved SoA (common.hpp, pop_index):
+// column x holds the nine q-segments of the reference's f(ld, nx, 0:8)
+// (src/fvm_bardow.F90:153) back to back, each ldd = ny rounded up to 64 elements long,
+// so every segment starts 128-byte aligned for FP16 and 512-byte aligned for FP64.
 //
 // Hot path: k_fused, one pull-stream + collide pass per step (lbm_stream,
 // src/periodic_lbm.f90:45-127, then collide_*, src/collision_*.F90).  Pull streaming
@@ -13,6 +13,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -81,48 +85,48 @@ __device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const 
 
 // Fused pull-stream + collide.  One thread per cell; a workgroup covers blockDim.x
 // consecutive y of one column x.  f_q(y, x) <- src_q(y - cy_q, x - cx_q).
-template <int MODEL, int ST, bool REMAP>
-__global__ void __launch_bounds__(256) k_fused(StepArgs a) {
+template <int MODEL, int ST>
+__global__ void __launch_bounds__(1024) k_fused(StepArgs a) {
     using T = typename Traits<ST>::T;
     using R = typename Traits<ST>::R;
-    const int b = REMAP ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int by = b % a.nby, x = b / a.nby;
     const int y = by * blockDim.x + threadIdx.x;
     if (y >= a.ny) return;
     const int nx = a.nx, ny = a.ny;
     const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
-    const size_t P = a.plane, L = (size_t)a.ldd;
-    const size_t c0 = (size_t)x * L, cm = (size_t)xm * L, cp = (size_t)xp * L;
+    const size_t L = (size_t)a.ldd;
+    const size_t c0 = (size_t)x * a.cs, cm = (size_t)xm * a.cs, cp = (size_t)xp * a.cs;
     const T *__restrict__ s = static_cast<const T *>(a.src);
     T *__restrict__ d = static_cast<T *>(a.dst);
     const float is = a.inv_scale;
     R f[9];
-    f[0] = load_pop<ST>(s + 0 * P + c0 + y, is);
-    f[1] = load_pop<ST>(s + 1 * P + cm + y, is);
-    f[2] = load_pop<ST>(s + 2 * P + c0 + ym, is);
-    f[3] = load_pop<ST>(s + 3 * P + cp + y, is);
-    f[4] = load_pop<ST>(s + 4 * P + c0 + yp, is);
-    f[5] = load_pop<ST>(s + 5 * P + cm + ym, is);
-    f[6] = load_pop<ST>(s + 6 * P + cp + ym, is);
-    f[7] = load_pop<ST>(s + 7 * P + cp + yp, is);
-    f[8] = load_pop<ST>(s + 8 * P + cm + yp, is);
+    f[0] = load_pop<ST>(s + c0 + 0 * L + y, is);
+    f[1] = load_pop<ST>(s + cm + 1 * L + y, is);
+    f[2] = load_pop<ST>(s + c0 + 2 * L + ym, is);
+    f[3] = load_pop<ST>(s + cp + 3 * L + y, is);
+    f[4] = load_pop<ST>(s + c0 + 4 * L + yp, is);
+    f[5] = load_pop<ST>(s + cm + 5 * L + ym, is);
+    f[6] = load_pop<ST>(s + cp + 6 * L + ym, is);
+    f[7] = load_pop<ST>(s + cp + 7 * L + yp, is);
+    f[8] = load_pop<ST>(s + cm + 8 * L + yp, is);
     collide_cell<MODEL, ST>(f, a);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * P + c0 + y, f[q], a.scale);
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + c0 + q * L + y, f[q], a.scale);
 }
 
 // lbm_stream alone (src/periodic_lbm.f90:45-127): raw copy of stored values.
 template <class T>
 __global__ void __launch_bounds__(256) k_stream(const T *__restrict__ s, T *__restrict__ d,
-                                                int nx, int ny, int ldd, size_t P) {
+                                                int nx, int ny, int ldd) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int x = blockIdx.y;
     if (y >= ny) return;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const int xs = (x - cx(q) + nx) % nx, ys = (y - cy(q) + ny) % ny;
-        d[q * P + (size_t)x * ldd + y] = s[q * P + (size_t)xs * ldd + ys];
+        d[pop_index(q, x, y, ldd)] = s[pop_index(q, xs, ys, ldd)];
     }
 }
 
@@ -135,24 +139,25 @@ __global__ void __launch_bounds__(256) k_collide(StepArgs a) {
     const int x = blockIdx.y;
     if (y >= a.ny) return;
     T *d = static_cast<T *>(a.dst);
-    const size_t c = (size_t)x * a.ldd + y;
+    const size_t c = (size_t)x * a.cs + y;
     R f[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) f[q] = load_pop<ST>(d + q * a.plane + c, a.inv_scale);
+    for (int q = 0; q < 9; ++q) f[q] = load_pop<ST>(d + c + (size_t)q * a.ldd, a.inv_scale);
     collide_cell<MODEL, ST>(f, a);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * a.plane + c, f[q], a.scale);
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + c + (size_t)q * a.ldd, f[q], a.scale);
 }
 
 // Cell macros in FP64.  F64: update_macros order (src/fvm_bardow.F90:371-379).
 // Shifted: rho = 1 + sum g, u = j/rho.
 template <int ST>
-__device__ __forceinline__ void cell_macros(const typename Traits<ST>::T *s, size_t P, size_t c,
+__device__ __forceinline__ void cell_macros(const typename Traits<ST>::T *s, int ldd, int x, int y,
                                             float inv_scale, double &rho, double &ux,
                                             double &uy) {
     double f[9];
+    const size_t c = pop_index(0, x, y, ldd);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) f[q] = double(load_pop<ST>(s + q * P + c, inv_scale));
+    for (int q = 0; q < 9; ++q) f[q] = double(load_pop<ST>(s + c + (size_t)q * ldd, inv_scale));
     if constexpr (!Traits<ST>::shifted) {
         macros_ref(f, rho, ux, uy);
     } else {
@@ -166,14 +171,14 @@ __device__ __forceinline__ void cell_macros(const typename Traits<ST>::T *s, siz
 
 template <int ST>
 __global__ void __launch_bounds__(256) k_macros(const void *src, int nx, int ny, int ldd,
-                                                size_t P, float inv_scale, double *rho,
-                                                double *ux, double *uy) {
+                                                float inv_scale, double *rho, double *ux,
+                                                double *uy) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int x = blockIdx.y;
     if (y >= ny) return;
     const auto *s = static_cast<const typename Traits<ST>::T *>(src);
     double r, u, v;
-    cell_macros<ST>(s, P, (size_t)x * ldd + y, inv_scale, r, u, v);
+    cell_macros<ST>(s, ldd, x, y, inv_scale, r, u, v);
     const size_t o = (size_t)x * ny + y;
     rho[o] = r;
     ux[o] = u;
@@ -184,7 +189,7 @@ __global__ void __launch_bounds__(256) k_macros(const void *src, int nx, int ny,
 // column-major cell index), fixed LDS tree per block, one block for the final pass.
 template <int ST>
 __global__ void __launch_bounds__(256) k_reduce(const void *src, int nx, int ny, int ldd,
-                                                size_t P, float inv_scale, const double *uxa,
+                                                float inv_scale, const double *uxa,
                                                 const double *uya, double *partials) {
     __shared__ double sh[256][PLBM_NRED];
     const auto *s = static_cast<const typename Traits<ST>::T *>(src);
@@ -194,7 +199,7 @@ __global__ void __launch_bounds__(256) k_reduce(const void *src, int nx, int ny,
          i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i / ny), y = (int)(i % ny);
         double rho, ux, uy;
-        cell_macros<ST>(s, P, (size_t)x * ldd + y, inv_scale, rho, ux, uy);
+        cell_macros<ST>(s, ldd, x, y, inv_scale, rho, ux, uy);
         const double u2 = ux * ux + uy * uy;
         if (!(__builtin_isfinite(rho) && __builtin_isfinite(ux) && __builtin_isfinite(uy))) {
             acc[PLBM_RED_NONFINITE] += 1.0;
@@ -241,49 +246,50 @@ __global__ void __launch_bounds__(PLBM_NRED) k_reduce_final(const double *partia
 template <int ST>
 __global__ void __launch_bounds__(256) k_equilibrium(const double *rho, const double *ux,
                                                      const double *uy, void *dst, int nx,
-                                                     int ny, int ldd, size_t P, float scale) {
+                                                     int ny, int ldd, float scale) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int x = blockIdx.y;
     if (y >= ny) return;
     auto *d = static_cast<typename Traits<ST>::T *>(dst);
-    const size_t i = (size_t)x * ny + y, c = (size_t)x * ldd + y;
+    const size_t i = (size_t)x * ny + y, c = pop_index(0, x, y, ldd);
     double feq[9];
     if constexpr (!Traits<ST>::shifted)
         equilibrium_ref(rho[i], ux[i], uy[i], feq);
     else
         equilibrium_shifted<double>(rho[i] - 1.0, ux[i], uy[i], feq);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) store_pop<ST>(d + q * P + c, feq[q], scale);
+    for (int q = 0; q < 9; ++q) store_pop<ST>(d + c + (size_t)q * ldd, feq[q], scale);
 }
 
 // One q-plane between host layout (FP64, unshifted, pitch ld) and storage.
 template <int ST>
 __global__ void __launch_bounds__(256) k_plane_from_f64(const double *src, int ld, void *dst,
-                                                        int nx, int ny, int ldd, double wshift,
-                                                        float scale) {
+                                                        int q, int nx, int ny, int ldd,
+                                                        double wshift, float scale) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int x = blockIdx.y;
     if (y >= ny) return;
     auto *d = static_cast<typename Traits<ST>::T *>(dst);
     const double v = src[(size_t)x * ld + y];
     using T = typename Traits<ST>::T;
+    const size_t o = pop_index(q, x, y, ldd);
     if constexpr (ST == PLBM_F16S)
-        d[(size_t)x * ldd + y] = T(float((v - wshift) * double(scale)));
+        d[o] = T(float((v - wshift) * double(scale)));
     else if constexpr (Traits<ST>::shifted)
-        d[(size_t)x * ldd + y] = T(v - wshift);
+        d[o] = T(v - wshift);
     else
-        d[(size_t)x * ldd + y] = T(v);
+        d[o] = T(v);
 }
 
 template <int ST>
-__global__ void __launch_bounds__(256) k_plane_to_f64(const void *src, int nx, int ny, int ldd,
-                                                      double *dst, int ld, double wshift,
+__global__ void __launch_bounds__(256) k_plane_to_f64(const void *src, int q, int nx, int ny,
+                                                      int ldd, double *dst, int ld, double wshift,
                                                       float inv_scale) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int x = blockIdx.y;
     if (y >= ny) return;
     const auto *s = static_cast<const typename Traits<ST>::T *>(src);
-    const double v = double(s[(size_t)x * ldd + y]);
+    const double v = double(s[pop_index(q, x, y, ldd)]);
     if constexpr (ST == PLBM_F16S)
         dst[(size_t)x * ld + y] = v * double(inv_scale) + wshift;
     else if constexpr (Traits<ST>::shifted)
@@ -292,13 +298,13 @@ __global__ void __launch_bounds__(256) k_plane_to_f64(const void *src, int nx, i
         dst[(size_t)x * ld + y] = v;
 }
 
-// HBM probe: 16-byte vector copy, grid-stride.
+// HBM probe: one 16-byte vector per thread, non-temporal store, one pass over the
+// array with many small workgroups (the fastest plain copy shape measured on MI355X).
 typedef float v4f __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_copy(const v4f *__restrict__ s, v4f *__restrict__ d,
                                               size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(s[i], d + i);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(s[i], d + i);
 }
 
 // ---------------------------------------------------------------------------- dispatch
@@ -326,15 +332,17 @@ static int dispatch_model(int m, F &&f) {
     return PLBM_ERR_ARG;
 }
 
-static StepArgs make_args(const plbm_lattice *h, const void *src, void *dst) {
+static StepArgs make_args(const plbm_lattice *h, const void *src, void *dst,
+                          LaunchCfg cfg = LaunchCfg()) {
     StepArgs a;
     a.src = src;
     a.dst = dst;
     a.nx = h->nx;
     a.ny = h->ny;
     a.ldd = h->ldd;
-    a.nby = (h->ny + h->block - 1) / h->block;
-    a.plane = h->plane;
+    a.nby = (h->ny + cfg.block - 1) / cfg.block;
+    a.cs = (size_t)9 * h->ldd;
+    a.remap = cfg.remap;
     a.omega = h->omega;
     a.lambda_odd = h->lambda_odd;
     a.omega_f = float(h->omega);
@@ -370,21 +378,118 @@ static dim3 cell_grid(const plbm_lattice *h) {
     return dim3((unsigned)((h->ny + 255) / 256), (unsigned)h->nx);
 }
 
-int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
-    StepArgs a = make_args(h, src, dst);
+int launch_step_cfg(plbm_lattice *h, int model, const void *src, void *dst, LaunchCfg cfg) {
+    StepArgs a = make_args(h, src, dst, cfg);
     const unsigned nblocks = (unsigned)((size_t)a.nby * h->nx);
     return dispatch_model(model, [&](auto M) {
         return dispatch_storage(h->storage, [&](auto S) {
-            if (h->xcd_remap)
-                hipLaunchKernelGGL((k_fused<decltype(M)::value, decltype(S)::value, true>),
-                                   dim3(nblocks), dim3(h->block), 0, h->stream, a);
-            else
-                hipLaunchKernelGGL((k_fused<decltype(M)::value, decltype(S)::value, false>),
-                                   dim3(nblocks), dim3(h->block), 0, h->stream, a);
+            hipLaunchKernelGGL((k_fused<decltype(M)::value, decltype(S)::value>), dim3(nblocks),
+                               dim3(cfg.block), 0, h->stream, a);
             PLBM_HIP(hipGetLastError());
             return PLBM_OK;
         });
     });
+}
+
+int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG,
+                 "unknown collision model " + std::to_string(model));
+    return launch_step_cfg(h, model, src, dst, h->cfg[model]);
+}
+
+static bool valid_cfg(LaunchCfg c) {
+    return c.block >= 64 && c.block <= 1024 && c.block % 64 == 0 && (c.remap == 0 || c.remap == 1);
+}
+
+// ---------------------------------------------------------------------------- autotuner
+//
+// Picks the fused kernel's launch shape (workgroup size x XCD renumbering) for one
+// (device, storage, model, grid) by timing every candidate with HIP events on the
+// lattice's stream: `reps` launches from buffer src into a scratch buffer (so the
+// lattice's own state is untouched), best of three trials each.  Results are cached per
+// process, keyed by device name, storage, model and grid size.
+
+struct TuneKey {
+    std::string dev;
+    int storage, model, nx, ny;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(dev, storage, model, nx, ny) <
+               std::tie(o.dev, o.storage, o.model, o.nx, o.ny);
+    }
+};
+static std::map<TuneKey, std::pair<LaunchCfg, double>> g_tune_cache;
+static std::mutex g_tune_mu;
+
+static std::string device_name(int dev) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+    return std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount);
+}
+
+int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best_out,
+                  double *best_ms) {
+    TuneKey key{device_name(h->device), h->storage, model, h->nx, h->ny};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            *best_out = it->second.first;
+            *best_ms = it->second.second;
+            return PLBM_OK;
+        }
+    }
+    void *scratch = nullptr;
+    const size_t bytes = h->elems * storage_bytes(h->storage);
+    if (hipMalloc(&scratch, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("autotune: no memory for a scratch buffer");
+        return PLBM_ERR_NOMEM;
+    }
+    hipEvent_t e0, e1;
+    PLBM_HIP(hipEventCreate(&e0));
+    PLBM_HIP(hipEventCreate(&e1));
+    LaunchCfg best;
+    double tbest = 1e300;
+    int status = PLBM_OK;
+    for (int block : {64, 128, 256, 512, 1024}) {
+        for (int remap : {0, 1}) {
+            LaunchCfg c{block, remap};
+            double tmin = 1e300;
+            for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
+                status = launch_step_cfg(h, model, h->f[src], scratch, c);  // warm
+                if (status != PLBM_OK) break;
+                (void)hipEventRecord(e0, h->stream);
+                for (int r = 0; r < reps && status == PLBM_OK; ++r)
+                    status = launch_step_cfg(h, model, h->f[src], scratch, c);
+                (void)hipEventRecord(e1, h->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("autotune: event synchronisation failed");
+                    status = PLBM_ERR_HIP;
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                tmin = std::min(tmin, double(ms) / reps);
+            }
+            if (status != PLBM_OK) break;
+            if (tmin < tbest) {
+                tbest = tmin;
+                best = c;
+            }
+        }
+        if (status != PLBM_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (status != PLBM_OK) return status;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = {best, tbest};
+    }
+    *best_out = best;
+    *best_ms = tbest;
+    return PLBM_OK;
 }
 
 static int require_params(const plbm_lattice *h) {
@@ -421,15 +526,14 @@ int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) {
                  "nf must be in [1," + std::to_string(kMaxBuffers) + "]");
     PLBM_REQUIRE(storage >= PLBM_F64 && storage <= PLBM_F16S, PLBM_ERR_ARG, "unknown storage");
     const int ldd = (ny + 63) / 64 * 64;
-    PLBM_REQUIRE((size_t)nx * ldd < ((size_t)1 << 31), PLBM_ERR_ARG,
-                 "plane of nx*ldd elements must stay below 2^31");
+    PLBM_REQUIRE((size_t)nx < ((size_t)1 << 31) / 9, PLBM_ERR_ARG, "nx too large");
     auto *h = new plbm_lattice();
     h->nx = nx;
     h->ny = ny;
     h->ldd = ldd;
     h->nf = nf;
     h->storage = storage;
-    h->plane = (size_t)nx * ldd;
+    h->elems = (size_t)9 * nx * ldd;
     h->scale = storage == PLBM_F16S ? 1024.0 : 1.0;
     if (hipGetDevice(&h->device) != hipSuccess ||
         hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess) {
@@ -439,7 +543,7 @@ int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) {
         return PLBM_ERR_HIP;
     }
     h->stream = h->own;
-    const size_t bytes = 9 * h->plane * storage_bytes(storage);
+    const size_t bytes = h->elems * storage_bytes(storage);
     for (int b = 0; b < nf; ++b) {
         if (hipMalloc(&h->f[b], bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -478,7 +582,7 @@ int plbm_get_info(const plbm_lattice *h, plbm_info *out) {
     out->trt_magic = h->trt_magic;
     out->lambda_odd = h->lambda_odd;
     out->f16_scale = h->scale;
-    out->bytes_per_buffer = 9 * h->plane * storage_bytes(h->storage);
+    out->bytes_per_buffer = h->elems * storage_bytes(h->storage);
     return PLBM_OK;
 }
 
@@ -520,16 +624,14 @@ int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) {
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
     PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
-    const size_t eb = storage_bytes(h->storage);
     for (int q = 0; q < 9; ++q) {
         // host plane (pitch ld) -> compact staging plane (pitch ny)
         PLBM_HIP(hipMemcpy2DAsync(h->d_tmp, (size_t)h->ny * 8, f + q * plane_host, (size_t)ld * 8,
                                   (size_t)h->ny * 8, h->nx, hipMemcpyHostToDevice, h->stream));
-        void *dst = static_cast<char *>(h->f[buf]) + q * h->plane * eb;
         const double w = wq(q);
         PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
             hipLaunchKernelGGL(k_plane_from_f64<decltype(S)::value>, cell_grid(h), dim3(256), 0,
-                               h->stream, h->d_tmp, h->ny, dst, h->nx, h->ny, h->ldd, w,
+                               h->stream, h->d_tmp, h->ny, h->f[buf], q, h->nx, h->ny, h->ldd, w,
                                float(h->scale));
             PLBM_HIP(hipGetLastError());
             return PLBM_OK;
@@ -544,13 +646,11 @@ int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
     PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
-    const size_t eb = storage_bytes(h->storage);
     for (int q = 0; q < 9; ++q) {
-        const void *src = static_cast<const char *>(h->f[buf]) + q * h->plane * eb;
         const double w = wq(q);
         PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
             hipLaunchKernelGGL(k_plane_to_f64<decltype(S)::value>, cell_grid(h), dim3(256), 0,
-                               h->stream, src, h->nx, h->ny, h->ldd, h->d_tmp, h->ny, w,
+                               h->stream, h->f[buf], q, h->nx, h->ny, h->ldd, h->d_tmp, h->ny, w,
                                float(1.0 / h->scale));
             PLBM_HIP(hipGetLastError());
             return PLBM_OK;
@@ -569,7 +669,7 @@ int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     return dispatch_storage(h->storage, [&](auto S) {
         hipLaunchKernelGGL(k_equilibrium<decltype(S)::value>, cell_grid(h), dim3(256), 0,
-                           h->stream, rho, ux, uy, h->f[buf], h->nx, h->ny, h->ldd, h->plane,
+                           h->stream, rho, ux, uy, h->f[buf], h->nx, h->ny, h->ldd,
                            float(h->scale));
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
@@ -620,7 +720,7 @@ int plbm_stream(plbm_lattice *h, int src, int dst) {
         using T = typename Traits<decltype(S)::value>::T;
         hipLaunchKernelGGL(k_stream<T>, cell_grid(h), dim3(256), 0, h->stream,
                            static_cast<const T *>(h->f[src]), static_cast<T *>(h->f[dst]), h->nx,
-                           h->ny, h->ldd, h->plane);
+                           h->ny, h->ldd);
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
     });
@@ -630,6 +730,8 @@ int plbm_collide(plbm_lattice *h, int model, int buf) {
     PLBM_TRY(check_buf(h, buf));
     PLBM_TRY(require_params(h));
     StepArgs a = make_args(h, h->f[buf], h->f[buf]);
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG,
+                 "unknown collision model " + std::to_string(model));
     return dispatch_model(model, [&](auto M) {
         return dispatch_storage(h->storage, [&](auto S) {
             hipLaunchKernelGGL((k_collide<decltype(M)::value, decltype(S)::value>), cell_grid(h),
@@ -644,7 +746,7 @@ int plbm_copy(plbm_lattice *h, int src, int dst) {
     PLBM_TRY(check_buf(h, src));
     PLBM_TRY(check_buf(h, dst));
     if (src == dst) return PLBM_OK;
-    PLBM_HIP(hipMemcpyAsync(h->f[dst], h->f[src], 9 * h->plane * storage_bytes(h->storage),
+    PLBM_HIP(hipMemcpyAsync(h->f[dst], h->f[src], h->elems * storage_bytes(h->storage),
                             hipMemcpyDeviceToDevice, h->stream));
     return PLBM_OK;
 }
@@ -654,7 +756,7 @@ int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *u
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     return dispatch_storage(h->storage, [&](auto S) {
         hipLaunchKernelGGL(k_macros<decltype(S)::value>, cell_grid(h), dim3(256), 0, h->stream,
-                           h->f[buf], h->nx, h->ny, h->ldd, h->plane, float(1.0 / h->scale), rho,
+                           h->f[buf], h->nx, h->ny, h->ldd, float(1.0 / h->scale), rho,
                            ux, uy);
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
@@ -686,7 +788,7 @@ int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *u
     const int nb = (int)std::min<size_t>(kReduceBlocks, (n + 255) / 256);
     PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
         hipLaunchKernelGGL(k_reduce<decltype(S)::value>, dim3(nb), dim3(256), 0, h->stream,
-                           h->f[buf], h->nx, h->ny, h->ldd, h->plane, float(1.0 / h->scale), uxa,
+                           h->f[buf], h->nx, h->ny, h->ldd, float(1.0 / h->scale), uxa,
                            uya, h->d_partials);
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
@@ -718,6 +820,37 @@ int plbm_sync(plbm_lattice *h) {
     return PLBM_OK;
 }
 
+int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    LaunchCfg c{block, xcd_remap};
+    PLBM_REQUIRE(valid_cfg(c), PLBM_ERR_ARG,
+                 "block must be a multiple of 64 in [64,1024] and xcd_remap 0 or 1");
+    h->cfg[model] = c;
+    return PLBM_OK;
+}
+
+int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap) {
+    PLBM_REQUIRE(h && block && xcd_remap, PLBM_ERR_ARG, "null argument");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    *block = h->cfg[model].block;
+    *xcd_remap = h->cfg[model].remap;
+    return PLBM_OK;
+}
+
+int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_TRY(require_params(h));
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(reps >= 1, PLBM_ERR_ARG, "reps must be >= 1");
+    LaunchCfg best;
+    double ms = 0;
+    PLBM_TRY(autotune_step(h, model, buf, reps, &best, &ms));
+    h->cfg[model] = best;
+    if (best_ms) *best_ms = ms;
+    return PLBM_OK;
+}
+
 int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
     PLBM_REQUIRE(ms && iters > 0 && bytes >= 16, PLBM_ERR_ARG, "bad argument");
     const size_t n = bytes / 16;
@@ -732,10 +865,8 @@ int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
     PLBM_HIP(hipEventCreate(&e0));
     PLBM_HIP(hipEventCreate(&e1));
     PLBM_HIP(hipMemset(a, 0, n * 16));
-    int dev = 0, ncu = 0;
-    PLBM_HIP(hipGetDevice(&dev));
-    PLBM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const unsigned grid = (unsigned)std::max(1, ncu) * 32;
+    PLBM_REQUIRE(n / 256 < (size_t)1 << 31, PLBM_ERR_ARG, "copy probe size too large");
+    const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, a, b, n);  // warm-up
     PLBM_HIP(hipEventRecord(e0, nullptr));
     for (int i = 0; i < iters; ++i)

@@ -81,6 +81,23 @@ class Lattice:
         check(lib().plbm_buffer_ptr(self.h, buf, ctypes.byref(p)))
         return p.value
 
+    # --- launch shape
+    def set_launch(self, model: str | int, block: int, xcd_remap: int = 1):
+        check(lib().plbm_set_launch(self.h, _abi.MODEL_IDS.get(model, model), block, xcd_remap))
+
+    def get_launch(self, model: str | int) -> tuple[int, int]:
+        b, r = ctypes.c_int(), ctypes.c_int()
+        check(lib().plbm_get_launch(self.h, _abi.MODEL_IDS.get(model, model), ctypes.byref(b),
+                                    ctypes.byref(r)))
+        return b.value, r.value
+
+    def autotune(self, model: str | int, buf: int = 0, reps: int = 10) -> float:
+        """Time every launch shape of the fused step; keep the fastest.  Returns ms/step."""
+        ms = ctypes.c_double()
+        check(lib().plbm_autotune(self.h, _abi.MODEL_IDS.get(model, model), buf, reps,
+                                  ctypes.byref(ms)))
+        return ms.value
+
     # --- populations (0-based buffer ids)
     def upload_f(self, buf: int, f: np.ndarray):
         f = np.asfortranarray(f, dtype=np.float64)

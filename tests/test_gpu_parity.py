@@ -210,3 +210,38 @@ def test_errors_are_reported(P, gpu):
         lat.step("bgk", 1, 1)
     with pytest.raises(P.PlbmError, match="unknown collision"):
         lat.step(9, 0, 1)
+
+
+@pytest.mark.parametrize("block,remap", [(64, 0), (128, 1), (512, 0), (1024, 1)])
+def test_launch_shapes_bit_exact(P, gpu, oracle, block, remap):
+    """Every launch shape the autotuner may pick computes the same cells (ragged grid)."""
+    nx, ny, steps = 45, 200, 6
+    print(f"case launch block={block} remap={remap}", flush=True)
+    rho, ux, uy = random_fields(nx, ny, 21)
+    lat = P.Lattice(nx, ny, 2, "f64")
+    lat.set_params(1.6, 0.25)
+    lat.set_launch("rr", block, remap)
+    assert lat.get_launch("rr") == (block, remap)
+    lat.equilibrium_init(0, rho, ux, uy)
+    iold, _ = lat.run("rr", steps, 0, 1)
+    latest, _ = oracle_run(oracle, "rr", nx, ny, rho, ux, uy, steps, 1.6)
+    np.testing.assert_array_equal(lat.download_f(iold)[:ny], latest[:ny])
+
+
+def test_autotune_keeps_state_and_results(P, gpu, oracle):
+    nx, ny = 300, 260
+    rho, ux, uy = random_fields(nx, ny, 22)
+    lat = P.Lattice(nx, ny, 2, "f64")
+    lat.set_params(1.9, 0.25)
+    lat.equilibrium_init(0, rho, ux, uy)
+    before = lat.download_f(0)
+    ms = lat.autotune("trt", 0, reps=2)
+    assert ms > 0
+    block, remap = lat.get_launch("trt")
+    assert block % 64 == 0 and 64 <= block <= 1024 and remap in (0, 1)
+    np.testing.assert_array_equal(lat.download_f(0), before)  # tuner wrote only scratch
+    iold, _ = lat.run("trt", 5, 0, 1)
+    latest, _ = oracle_run(oracle, "trt", nx, ny, rho, ux, uy, 5, 1.9)
+    np.testing.assert_array_equal(lat.download_f(iold)[:ny], latest[:ny])
+    with pytest.raises(P.PlbmError):
+        lat.set_launch("trt", 96, 0)
