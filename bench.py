@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--acc-steps", type=int, default=5000,
                     help="steps of the FP64 vs FP32/FP16 accuracy comparison (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--n", type=int, default=N, help="grid size (default 8192)")
     return ap.parse_args()
 

@@ -1,0 +1,134 @@
+"""Summarise rocprofv3 CSV output into the short text records kept under profiles/.
+
+  python periodic-lbm_amd/rocprof_summary.py stats  <kernel_stats.csv> <out.md> --cmd "<cmd>"
+  python periodic-lbm_amd/rocprof_summary.py trace  <kernel_trace.csv> <out.md>
+         --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N --cmd "<cmd>"
+  python periodic-lbm_amd/rocprof_summary.py traffic <fetch_counter_collection.csv>
+         <write_counter_collection.csv> <out.json> --kernel <substring> --cells N
+         --bytes-per-lu B --cmd "<cmd>"
+
+The traffic record calibrates FETCH_SIZE / WRITE_SIZE (KiB per dispatch) against the
+engine's copy probe (k_copy, which reads and writes a known number of bytes in the same
+process) and reports HBM bytes per launch of the selected kernel beside its algorithmic
+bytes (cells x bytes per lattice update).
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import statistics
+from pathlib import Path
+
+
+def _rows(path):
+    with open(path, newline="") as fh:
+        return list(csv.DictReader(fh))
+
+
+def stats(args):
+    rows = _rows(args.csv)
+    lines = [f"# rocprofv3 kernel statistics", "", f"Command: `{args.cmd}`", "",
+             "| kernel | calls | avg us | min us | max us | share % |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                     f"{float(r['Percentage']):.2f} |")
+    Path(args.out).write_text("\n".join(lines) + "\n")
+
+
+def trace(args):
+    """Per-kernel durations from kernel_trace.csv, restricted to the last --last dispatches
+    of each selected kernel (the timed region of bench.py, after warm-up and tuning)."""
+    rows = _rows(args.csv)
+    lines = [f"# rocprofv3 kernel trace: timed-region dispatches", "", f"Command: `{args.cmd}`", "",
+             f"Last {args.last} dispatches of each kernel below (the timed steps).", "",
+             "| kernel | dispatches | avg us | min us | max us | bytes/launch | GB/s at avg |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for spec in args.kernel:
+        sub, nbytes = spec.rsplit("=", 1)
+        d = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+              r["Kernel_Name"]) for r in rows if sub in r["Kernel_Name"]]
+        d.sort()
+        sel = [x[1] for x in d[-args.last:]]
+        if not sel:
+            raise SystemExit(f"no dispatch of {sub!r}")
+        avg = sum(sel) / len(sel)
+        nb = float(nbytes)
+        lines.append(f"| `{d[-1][2][:70]}` | {len(sel)} | {avg / 1e3:.1f} | {min(sel) / 1e3:.1f} | "
+                     f"{max(sel) / 1e3:.1f} | {nb:.4g} | {nb / avg:.1f} |")
+    Path(args.out).write_text("\n".join(lines) + "\n")
+
+
+def _per_kernel(rows, counter):
+    agg = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        agg.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return agg
+
+
+def _pick(agg, sub):
+    keys = [k for k in agg if sub in k]
+    if not keys:
+        raise SystemExit(f"no kernel matching {sub!r} in {list(agg)}")
+    return keys[0], statistics.median(agg[keys[0]])
+
+
+def traffic(args):
+    fetch = _per_kernel(_rows(args.fetch), "FETCH_SIZE")
+    write = _per_kernel(_rows(args.write), "WRITE_SIZE")
+    copy_name, copy_fetch = _pick(fetch, "k_copy")
+    _, copy_write = _pick(write, "k_copy")
+    copy_bytes = args.copy_bytes
+    f_scale = copy_bytes / (copy_fetch * 1024.0)
+    w_scale = copy_bytes / (copy_write * 1024.0)
+    kname, kfetch = _pick(fetch, args.kernel)
+    _, kwrite = _pick(write, args.kernel)
+    read_b = kfetch * 1024.0 * f_scale
+    write_b = kwrite * 1024.0 * w_scale
+    algo = args.cells * args.bytes_per_lu
+    rec = {
+        "kernel": kname, "command": args.cmd,
+        "fetch_kib_per_dispatch": kfetch, "write_kib_per_dispatch": kwrite,
+        "calibration": {"kernel": copy_name, "bytes_read_and_written": copy_bytes,
+                        "fetch_kib": copy_fetch, "write_kib": copy_write,
+                        "fetch_scale": f_scale, "write_scale": w_scale},
+        "read_bytes_per_launch": read_b, "write_bytes_per_launch": write_b,
+        "bytes_per_launch": read_b + write_b, "algorithmic_bytes_per_launch": algo,
+        "traffic_over_algorithmic": (read_b + write_b) / algo,
+    }
+    Path(args.out).write_text(json.dumps(rec, indent=1) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("out")
+    a.add_argument("--cmd", required=True)
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch")
+    b.add_argument("write")
+    b.add_argument("out")
+    b.add_argument("--kernel", required=True)
+    b.add_argument("--cells", type=float, required=True)
+    b.add_argument("--bytes-per-lu", type=float, required=True)
+    b.add_argument("--copy-bytes", type=float, default=float(4 << 30))
+    b.add_argument("--cmd", required=True)
+    c = sub.add_parser("trace")
+    c.add_argument("csv")
+    c.add_argument("out")
+    c.add_argument("--kernel", action="append", required=True,
+                   help="<name substring>=<algorithmic bytes per launch>, repeatable")
+    c.add_argument("--last", type=int, default=50)
+    c.add_argument("--cmd", required=True)
+    args = ap.parse_args()
+    {"stats": stats, "traffic": traffic, "trace": trace}[args.mode](args)
+
+
+if __name__ == "__main__":
+    main()
