@@ -175,6 +175,44 @@ int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap
  * process.  *best_ms (may be NULL) receives the winner's time per step. */
 int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms);
 
+/* ---------------------------------------------------------------- batched engine */
+
+/* Many independent periodic lattices of uneven size advanced together (no counterpart in
+ * the reference, which runs one grid per process; the per-case arithmetic is the same
+ * fused step as plbm_step).  Case c has an nx[c] x ny[c] grid, the batch one storage
+ * form and one collision model; per-case omega / trt_magic. */
+typedef struct plbm_batch plbm_batch;
+int plbm_batch_create(int ncases, const int *nx, const int *ny, int storage, int model,
+                      plbm_batch **out);
+int plbm_batch_destroy(plbm_batch *b);
+int plbm_batch_info(const plbm_batch *b, int *ncases, long long *cells, size_t *arena_bytes);
+/* omega[c]; trt_magic may be NULL (Lambda = 1/4 for every case). */
+int plbm_batch_set_params(plbm_batch *b, const double *omega, const double *trt_magic);
+/* Case c's current buffer = feq(rho, ux, uy), (ny, nx) Fortran-order host arrays. */
+int plbm_batch_equilibrium_init(plbm_batch *b, int c, const double *rho, const double *ux,
+                                const double *uy);
+/* Case c's current populations from/to the host layout f(ld, nx, 0:8), unshifted FP64. */
+int plbm_batch_upload_f(plbm_batch *b, int c, const double *f, int ld);
+int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld);
+/* rho and u = j/rho of case c's current state (host arrays (ny, nx)). */
+int plbm_batch_macros(plbm_batch *b, int c, double *rho, double *ux, double *uy);
+/* Per-case FP64 reductions of the current state: out[c*PLBM_NRED + PLBM_RED_*]. */
+int plbm_batch_reduce(plbm_batch *b, double *out);
+/* Advance every case nsteps steps: replays hipGraphs of up to 100 steps (tile-table step
+ * kernel per step + one persistent kernel for the small cases on a forked stream).
+ * Asynchronous; plbm_batch_run_direct does the same with plain launches. */
+int plbm_batch_run(plbm_batch *b, int nsteps);
+int plbm_batch_run_direct(plbm_batch *b, int nsteps);
+int plbm_batch_sync(plbm_batch *b);
+/* Launch plan: tiles of ty x tx cells (ty*tx a multiple of 64, <= 1024); cases with
+ * nx*ny <= resident_cells run in the persistent kernel with resident_block threads. */
+int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int resident_block);
+int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
+                          int *resident_block);
+/* Autotuner: time `steps` (even) steps of every plan candidate on a scratch copy of the
+ * arena and keep the fastest; *best_ms (may be NULL) = its time per step. */
+int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms);
+
 /* ---------------------------------------------------------------- measurement */
 
 /* HBM probe: time `iters` launches of a streaming copy of `bytes` bytes (read + write)

@@ -23,3 +23,6 @@ __all__ = [
     "set_pdf_to_equilibrium", "set_properties", "sync_device", "sync_host", "update_macros",
     "w",
 ]
+from .batch import Batch, Case, sweep_cases  # noqa: E402
+
+__all__ += ["Batch", "Case", "sweep_cases"]

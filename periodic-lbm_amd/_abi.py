@@ -81,6 +81,22 @@ SIGNATURES = {
     "plbm_set_launch": (_i, [_vp, _i, _i, _i]),
     "plbm_get_launch": (_i, [_vp, _i, _ip, _ip]),
     "plbm_autotune": (_i, [_vp, _i, _i, _i, _dp]),
+    "plbm_batch_create": (_i, [_i, _ip, _ip, _i, _i, ctypes.POINTER(_vp)]),
+    "plbm_batch_destroy": (_i, [_vp]),
+    "plbm_batch_info": (_i, [_vp, _ip, ctypes.POINTER(ctypes.c_longlong),
+                             ctypes.POINTER(_sz)]),
+    "plbm_batch_set_params": (_i, [_vp, _dp, _dp]),
+    "plbm_batch_equilibrium_init": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "plbm_batch_upload_f": (_i, [_vp, _i, _dp, _i]),
+    "plbm_batch_download_f": (_i, [_vp, _i, _dp, _i]),
+    "plbm_batch_macros": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "plbm_batch_reduce": (_i, [_vp, _dp]),
+    "plbm_batch_run": (_i, [_vp, _i]),
+    "plbm_batch_run_direct": (_i, [_vp, _i]),
+    "plbm_batch_sync": (_i, [_vp]),
+    "plbm_batch_set_launch": (_i, [_vp, _i, _i, _i, _i]),
+    "plbm_batch_get_launch": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "plbm_batch_autotune": (_i, [_vp, _i, _dp]),
 }
 
 _LIB = None

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "../../include/plbm.h"
 
@@ -94,6 +95,30 @@ struct StepArgs {
     float omega_f, lambda_odd_f;
     float scale, inv_scale; // FP16 storage scale (1 otherwise)
 };
+
+// Compile-time dispatch of a runtime storage / model id to a templated callable.
+template <class F>
+inline int dispatch_storage(int st, F &&f) {
+    switch (st) {
+    case PLBM_F64: return f(std::integral_constant<int, PLBM_F64>{});
+    case PLBM_F64S: return f(std::integral_constant<int, PLBM_F64S>{});
+    case PLBM_F32S: return f(std::integral_constant<int, PLBM_F32S>{});
+    case PLBM_F16S: return f(std::integral_constant<int, PLBM_F16S>{});
+    }
+    set_error("unknown storage id " + std::to_string(st));
+    return PLBM_ERR_ARG;
+}
+
+template <class F>
+inline int dispatch_model(int m, F &&f) {
+    switch (m) {
+    case PLBM_BGK: return f(std::integral_constant<int, PLBM_BGK>{});
+    case PLBM_TRT: return f(std::integral_constant<int, PLBM_TRT>{});
+    case PLBM_RR: return f(std::integral_constant<int, PLBM_RR>{});
+    }
+    set_error("unknown collision model " + std::to_string(m));
+    return PLBM_ERR_ARG;
+}
 
 constexpr int kMaxBuffers = 4;
 constexpr int kReduceBlocks = 1024;

@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "d2q9.hpp"
+#include "stepper.hpp"
 
 namespace plbm {
 
@@ -30,56 +31,6 @@ void set_error(const std::string &msg) { g_err = msg; }
 const char *last_error() { return g_err.c_str(); }
 
 // ---------------------------------------------------------------------------- helpers
-
-// Workgroup b of G is dispatched to XCD b % 8.  Give each XCD a contiguous range of
-// logical tiles so neighbouring y-tiles of a column (which share the cache line that a
-// y+-1 shifted load straddles) sit in the same L2.
-__device__ __forceinline__ int xcd_remap(int b, int G) {
-    constexpr int NX = 8;
-    const int q = G / NX, r = G % NX;
-    const int xcd = b % NX, idx = b / NX;
-    const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-    return base + idx;
-}
-
-template <int ST>
-__device__ __forceinline__ typename Traits<ST>::R load_pop(const typename Traits<ST>::T *p,
-                                                           float inv_scale) {
-    using R = typename Traits<ST>::R;
-    if constexpr (ST == PLBM_F16S)
-        return R(*p) * inv_scale;
-    else
-        return R(*p);
-}
-
-template <int ST>
-__device__ __forceinline__ void store_pop(typename Traits<ST>::T *p, typename Traits<ST>::R v,
-                                          float scale) {
-    using T = typename Traits<ST>::T;
-    if constexpr (ST == PLBM_F16S)
-        __builtin_nontemporal_store(T(v * scale), p);
-    else
-        __builtin_nontemporal_store(T(v), p);
-}
-
-template <int MODEL, int ST>
-__device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const StepArgs &a) {
-    using R = typename Traits<ST>::R;
-    if constexpr (!Traits<ST>::shifted) {
-        if constexpr (MODEL == PLBM_BGK) collide_bgk_ref(f, a.omega);
-        else if constexpr (MODEL == PLBM_TRT) collide_trt_ref(f, a.omega, a.lambda_odd);
-        else collide_rr_ref(f, a.omega);
-    } else {
-        const R om = sizeof(R) == 8 ? R(a.omega) : R(a.omega_f);
-        const R lo = sizeof(R) == 8 ? R(a.lambda_odd) : R(a.lambda_odd_f);
-        R d0, jx0, jy0;
-        if constexpr (sizeof(R) == 4) moments_shifted(f, d0, jx0, jy0);
-        if constexpr (MODEL == PLBM_BGK) collide_bgk_shifted<R>(f, om);
-        else if constexpr (MODEL == PLBM_TRT) collide_trt_shifted<R>(f, om, lo);
-        else collide_rr_shifted<R>(f, om);
-        if constexpr (sizeof(R) == 4) restore_conservation(f, d0, jx0, jy0);
-    }
-}
 
 // ---------------------------------------------------------------------------- kernels
 
@@ -308,29 +259,6 @@ __global__ void __launch_bounds__(256) k_copy(const v4f *__restrict__ s, v4f *__
 }
 
 // ---------------------------------------------------------------------------- dispatch
-
-template <class F>
-static int dispatch_storage(int st, F &&f) {
-    switch (st) {
-    case PLBM_F64: return f(std::integral_constant<int, PLBM_F64>{});
-    case PLBM_F64S: return f(std::integral_constant<int, PLBM_F64S>{});
-    case PLBM_F32S: return f(std::integral_constant<int, PLBM_F32S>{});
-    case PLBM_F16S: return f(std::integral_constant<int, PLBM_F16S>{});
-    }
-    set_error("unknown storage id " + std::to_string(st));
-    return PLBM_ERR_ARG;
-}
-
-template <class F>
-static int dispatch_model(int m, F &&f) {
-    switch (m) {
-    case PLBM_BGK: return f(std::integral_constant<int, PLBM_BGK>{});
-    case PLBM_TRT: return f(std::integral_constant<int, PLBM_TRT>{});
-    case PLBM_RR: return f(std::integral_constant<int, PLBM_RR>{});
-    }
-    set_error("unknown collision model " + std::to_string(m));
-    return PLBM_ERR_ARG;
-}
 
 static StepArgs make_args(const plbm_lattice *h, const void *src, void *dst,
                           LaunchCfg cfg = LaunchCfg()) {
