@@ -213,6 +213,33 @@ int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_c
  * arena and keep the fastest; *best_ms (may be NULL) = its time per step. */
 int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms);
 
+/* ---------------------------------------------------------------- adjoint */
+
+/* Discrete adjoint of nsteps BGK steps (no counterpart in the reference).  Forward:
+ * f^0 = feq(rho0, u0), f^{n+1} = collide_bgk(lbm_stream(f^n)) with the engine's fused
+ * kernel; loss L = 1/2 sum_cells |u(f^N) - u*|^2, u = j/rho.  States are checkpointed
+ * every checkpoint_every steps; the reverse sweep recomputes each segment and runs the
+ * fused reverse kernel.  storage: PLBM_F64, PLBM_F64S (FP64 adjoint) or PLBM_F32S (FP32
+ * forward and adjoint).  All field pointers are device pointers to (ny, nx) Fortran-order
+ * FP64 arrays; the initial fields must stay valid until the backward call. */
+typedef struct plbm_adjoint plbm_adjoint;
+int plbm_adjoint_create(int nx, int ny, int storage, int nsteps, int checkpoint_every,
+                        plbm_adjoint **out);
+int plbm_adjoint_destroy(plbm_adjoint *a);
+int plbm_adjoint_set_stream(plbm_adjoint *a, void *stream);
+int plbm_adjoint_forward_dev(plbm_adjoint *a, const double *rho0, const double *ux0,
+                             const double *uy0, double omega, const double *uxt,
+                             const double *uyt, double *loss);
+/* Gradients of the last forward's loss: dL/drho0 (may be NULL), dL/dux0, dL/duy0
+ * (device, (ny,nx)) and dL/domega (host scalar). */
+int plbm_adjoint_backward_dev(plbm_adjoint *a, double *grho0, double *gux0, double *guy0,
+                              double *gomega);
+
+/* Divergence-free band-limited velocity field (streamfunction modes 0 < |k| <= kmax with
+ * amplitude 1/|k|^2 and random phases from `seed`), scaled to max |u| = amp. */
+int plbm_field_band_limited(int nx, int ny, int kmax, double amp, uint64_t seed, double *ux,
+                            double *uy);
+
 /* ---------------------------------------------------------------- measurement */
 
 /* HBM probe: time `iters` launches of a streaming copy of `bytes` bytes (read + write)

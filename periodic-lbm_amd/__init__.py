@@ -26,3 +26,6 @@ __all__ = [
 from .batch import Batch, Case, sweep_cases  # noqa: E402
 
 __all__ += ["Batch", "Case", "sweep_cases"]
+from .adjoint import Adjoint, band_limited_field, velocity_misfit  # noqa: E402
+
+__all__ += ["Adjoint", "band_limited_field", "velocity_misfit"]

@@ -97,6 +97,12 @@ SIGNATURES = {
     "plbm_batch_set_launch": (_i, [_vp, _i, _i, _i, _i]),
     "plbm_batch_get_launch": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "plbm_batch_autotune": (_i, [_vp, _i, _dp]),
+    "plbm_adjoint_create": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
+    "plbm_adjoint_destroy": (_i, [_vp]),
+    "plbm_adjoint_set_stream": (_i, [_vp, _vp]),
+    "plbm_adjoint_forward_dev": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _vp, _dp]),
+    "plbm_adjoint_backward_dev": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "plbm_field_band_limited": (_i, [_i, _i, _i, _d, _u64, _dp, _dp]),
 }
 
 _LIB = None

@@ -3,8 +3,10 @@
 // Built with -ffp-contract=off: each expression follows the reference's evaluation
 // order so the fields equal the reference's bit for bit when both call the same libm
 // (glibc sqrt/sin/cos/exp/tanh).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/plbm.h"
 
@@ -65,6 +67,53 @@ double plbm_noise(uint64_t seed, uint64_t counter) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
     return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Divergence-free band-limited velocity: streamfunction
+//   psi(x, y) = sum_{0 < |k| <= kmax, half plane} cos(2 pi (m x + n y)/L + phi_k) / |k|^2
+// on cell centres, u = d psi/dy, v = -d psi/dx, scaled so that max |u| = amp.  Phases
+// phi_k = 2 pi U with U the generator's deviates of `seed` (one per mode, in the order
+// the modes are enumerated: m from -kmax to kmax, n from 0 to kmax).
+int plbm_field_band_limited(int nx, int ny, int kmax, double amp, uint64_t seed, double *ux,
+                            double *uy) {
+    if (nx < 1 || ny < 1 || kmax < 1 || !ux || !uy) return PLBM_ERR_ARG;
+    const double pi = 4.0 * std::atan(1.0);
+    struct Mode { double kx, ky, a, phi; };
+    std::vector<Mode> modes;
+    uint64_t counter = 0;
+    for (int m = -kmax; m <= kmax; ++m)
+        for (int n = 0; n <= kmax; ++n) {
+            if (m * m + n * n == 0 || m * m + n * n > kmax * kmax) continue;
+            if (n == 0 && m < 0) continue;  // half plane: (m, n) and (-m, -n) are one mode
+            const double k2 = double(m * m + n * n);
+            modes.push_back({2.0 * pi * m / nx, 2.0 * pi * n / ny, 1.0 / k2,
+                             2.0 * pi * plbm_noise(seed, counter++)});
+        }
+    double umax = 0.0;
+    #pragma omp parallel for schedule(static) reduction(max : umax)
+    for (int x = 0; x < nx; ++x) {
+        const double xx = double(x) + 0.5;
+        for (int y = 0; y < ny; ++y) {
+            const double yy = double(y) + 0.5;
+            double u = 0.0, v = 0.0;
+            for (const Mode &md : modes) {
+                const double s = std::sin(md.kx * xx + md.ky * yy + md.phi);
+                u += -md.a * md.ky * s;  // d/dy cos(...) = -ky sin(...)
+                v += md.a * md.kx * s;   // -d/dx cos(...) = kx sin(...)
+            }
+            const size_t c = (size_t)x * ny + y;
+            ux[c] = u;
+            uy[c] = v;
+            umax = std::max(umax, std::sqrt(u * u + v * v));
+        }
+    }
+    const double s = umax > 0 ? amp / umax : 0.0;
+    const size_t n = (size_t)nx * ny;
+    for (size_t c = 0; c < n; ++c) {
+        ux[c] *= s;
+        uy[c] *= s;
+    }
+    return PLBM_OK;
 }
 
 int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy) {
