@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--n", type=int, default=N, help="grid size (default 8192)")
     ap.add_argument("--batch-steps", type=int, default=2000,
                     help="steps of the batched uneven-size sweep (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     return ap.parse_args()
 
 
@@ -129,8 +130,11 @@ def gpu_leg(args):
         torch.cuda.empty_cache()
 
     copy_gbs = P.copy_bandwidth(4 << 30, 20)
-    batched = batch_leg(P, torch, args) if args.batch_steps > 0 else None
-    return results, copy_gbs, dev_name, batched
+    extras = {
+        "batched_sweep": batch_leg(P, torch, args) if args.batch_steps > 0 else None,
+        "adjoint": None if args.no_adjoint else adjoint_leg(P, torch, args),
+    }
+    return results, copy_gbs, dev_name, extras
 
 
 def batch_leg(P, torch, args):
@@ -174,6 +178,50 @@ def batch_leg(P, torch, args):
     return out
 
 
+def adjoint_leg(P, torch, args):
+    """Adjoint inversion setup: 512^2 BGK, tau = 0.6, 500 steps, checkpoint every 50,
+    band-limited initial velocity (|k| <= 8, amplitude 0.02, seed 1), target from seed 2.
+    Times forward + reverse sweeps (FP32 and FP64) and checks <grad, d> against FP64
+    central differences (h = 1e-6) on 16 random directions and for tau."""
+    import numpy as np
+    n, steps, every, tau, h = 512, 500, 50, 0.6, 1e-6
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(np.ravel(a, order="F"))).cuda().view(n, n)
+    ux0, uy0 = (t(a) for a in P.band_limited_field(n, n, 8, 0.02, 1))
+    uxt, uyt = (t(a) for a in P.band_limited_field(n, n, 8, 0.02, 2))
+    rho0 = torch.ones_like(ux0)
+    out = {"grid": [n, n], "steps": steps, "checkpoint_every": every, "tau": tau}
+    eng64 = P.Adjoint(n, n, steps, every, "f64")
+    gen = torch.Generator(device="cpu").manual_seed(123)
+    dirs = [(torch.randn(n, n, generator=gen, dtype=torch.float64).cuda(),
+             torch.randn(n, n, generator=gen, dtype=torch.float64).cuda()) for _ in range(16)]
+    fds = []
+    for dx, dy in dirs:
+        lp = eng64.forward(rho0, (ux0 + h * dx).contiguous(), (uy0 + h * dy).contiguous(),
+                           1.0 / tau, uxt, uyt)
+        lm = eng64.forward(rho0, (ux0 - h * dx).contiguous(), (uy0 - h * dy).contiguous(),
+                           1.0 / tau, uxt, uyt)
+        fds.append((lp - lm) / (2 * h))
+    fd_tau = (eng64.forward(rho0, ux0, uy0, 1.0 / (tau + h), uxt, uyt)
+              - eng64.forward(rho0, ux0, uy0, 1.0 / (tau - h), uxt, uyt)) / (2 * h)
+    for storage in ("f32s", "f64"):
+        eng = eng64 if storage == "f64" else P.Adjoint(n, n, steps, every, storage)
+        eng.forward(rho0, ux0, uy0, 1.0 / tau, uxt, uyt)  # warm-up
+        eng.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = eng.forward(rho0, ux0, uy0, 1.0 / tau, uxt, uyt)
+        t1 = time.perf_counter()
+        _, gux, guy, gom = eng.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        errs = [abs(float(torch.sum(gux * dx) + torch.sum(guy * dy)) - fd) / abs(fd)
+                for (dx, dy), fd in zip(dirs, fds)]
+        out[storage] = {"loss": loss, "forward_s": t1 - t0, "backward_s": t2 - t1,
+                        "max_rel_err_field_dirs_vs_fd64": max(errs),
+                        "rel_err_tau_vs_fd64": abs(-gom / tau ** 2 - fd_tau) / abs(fd_tau)}
+    return out
+
+
 def cpu_leg(args):
     """The reference Fortran (oracle/_ref, flang -O2 -fopenmp) on the host cores, over a
     bounded sample of the same workload: the 8192^2 TRT grid for --cpu-steps steps."""
@@ -188,7 +236,8 @@ def cpu_leg(args):
     if po.have_ref():
         R = po.RefLib()
         grid = R.grid(n, n, "trt")
-        grid.set_properties(NU, 1.0, MAGIC)
+        # the omega set_properties(NU, dt=1, MAGIC) computes, without its stdout print
+        grid.set_omega(1.0 / (3.0 * NU + 0.5), MAGIC)
         grid.set_macros(rho, ux, uy)
         grid.set_pdf_to_equilibrium()
         grid.step(1)  # warm-up
@@ -233,7 +282,7 @@ def main():
               "multi-GPU slab decomposition is design-only", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
-    results, copy_gbs, dev_name, batched = gpu_leg(args)
+    results, copy_gbs, dev_name, extras = gpu_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
 
     f64 = results["f64"]
@@ -272,7 +321,8 @@ def main():
             for k, v in results.items()},
         "f32s_speedup_vs_f64": round(results["f32s"]["mlups"] / f64["mlups"], 3),
         "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
-        "batched_sweep": batched,
+        "batched_sweep": extras["batched_sweep"],
+        "adjoint": extras["adjoint"],
     }
     print(json.dumps(line), flush=True)
 

@@ -176,6 +176,7 @@ class RefLib:
         L.ref_set_model.argtypes = [ctypes.c_void_p, _i]
         L.ref_set_properties.argtypes = [ctypes.c_void_p, _d, _d, _d, _i]
         L.ref_get_params.argtypes = [ctypes.c_void_p, _dp]
+        L.ref_set_omega.argtypes = [ctypes.c_void_p, _d, _d]
         L.ref_set_macros.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ref_get_macros.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ref_perform_lbm_step.argtypes = [ctypes.c_void_p, _i]
@@ -237,6 +238,9 @@ class RefGrid:
     def set_properties(self, nu, dt, magic=None):
         self.ref.lib.ref_set_properties(self.h, nu, dt, 0.0 if magic is None else magic,
                                         0 if magic is None else 1)
+
+    def set_omega(self, omega, magic):
+        self.ref.lib.ref_set_omega(self.h, omega, magic)
 
     def params(self):
         out = np.zeros(6)

@@ -95,6 +95,17 @@ contains
       end if
    end subroutine
 
+   subroutine ref_set_omega(h, omega, magic) bind(c, name="ref_set_omega")
+      ! the two collision parameters set_properties derives, assigned directly (no
+      ! "trt magic" print on stdout, for callers that own stdout)
+      type(c_ptr), value :: h
+      real(c_double), value :: omega, magic
+      type(grid_box), pointer :: p
+      p => unbox(h)
+      p%grid%omega = omega
+      p%grid%trt_magic = magic
+   end subroutine
+
    subroutine ref_get_params(h, out) bind(c, name="ref_get_params")
       ! out = [nu, dt, tau, omega, trt_magic, csqr]
       type(c_ptr), value :: h

@@ -29,3 +29,6 @@ __all__ += ["Batch", "Case", "sweep_cases"]
 from .adjoint import Adjoint, band_limited_field, velocity_misfit  # noqa: E402
 
 __all__ += ["Adjoint", "band_limited_field", "velocity_misfit"]
+from . import sim  # noqa: E402
+
+__all__ += ["sim"]

@@ -30,7 +30,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip"]
-CXX_SOURCES = ["fields.cpp"]
+CXX_SOURCES = ["fields.cpp", "sim_plugin.cpp"]
 # -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
 # reference is built without FMA); kernels that want FMA enable it locally by pragma.
 # Packed FP32 instructions are disabled for all device code.
@@ -108,6 +108,17 @@ def link_fortran_program(src: Path, out: Path, moddir: Path | None = None) -> Pa
     return out
 
 
+def build_fortran_tests(verbose: bool = True) -> None:
+    """The Fortran test programs under tests/fortran, linked against the shim."""
+    out = PKG / "_build" / "tests"
+    for name in ("shim_check", "outputs_check"):
+        src = ROOT / "tests" / "fortran" / f"{name}.f90"
+        if src.exists():
+            link_fortran_program(src, out / name, moddir=out / f"mod_{name}")
+            if verbose:
+                print(f"[build] {(out / name).relative_to(ROOT)}")
+
+
 def build_app(ref_root: Path, verbose: bool = True) -> Path | None:
     src = ref_root / "app" / "main_taylor_green.f90"
     if not src.exists():
@@ -134,6 +145,7 @@ def build_all(verbose: bool = True) -> None:
     build_hip(verbose)
     if shutil.which(FLANG) or Path(FLANG).exists():
         build_fortran(verbose)
+        build_fortran_tests(verbose)
         if ref_root is not None:
             build_app(ref_root, verbose)
     build_oracle(ref_root, verbose)

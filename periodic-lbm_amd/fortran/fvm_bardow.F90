@@ -552,10 +552,13 @@ contains
       call make_folder(grid, "output_grid")
       write(s1,'(I0)') grid%ny
       write(s2,'(I0)') grid%nx
+      ! header text and padding as written by the reference's npy writer (fortran-lang
+      ! stdlib save_npy, format 1.0): every shape entry followed by ", ", then spaces up to
+      ! the next 64-byte boundary (a full 64 when already aligned) and a newline
       dict = "{'descr': '<f8', 'fortran_order': True, 'shape': (" // trim(s1) // ", " // &
-             trim(s2) // ", 3), }"
-      hlen = len(dict) + 1
-      pad = mod(64 - mod(10 + hlen, 64), 64)
+             trim(s2) // ", 3, ), }"
+      hlen = len(dict)
+      pad = 64 - mod(hlen + 10 + 1, 64)
       header = dict // repeat(' ', pad) // achar(10)
       open(newunit=unit, file=output_name(grid, step, '.npy'), access='stream', &
          form='unformatted', status='replace')

@@ -140,11 +140,24 @@ def app():
                                                    "values": vals}, indent=1))
 
 
+def outputs():
+    """sha256 of the files tests/fortran/outputs_check.f90 writes through the reference's
+    output_vtk / output_npy / output_gnuplot (oracle/_ref/outputs_check_ref)."""
+    exe = po.HERE / "_ref" / "outputs_check_ref"
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, check=True)
+        rec = {p.name: hashlib.sha256(p.read_bytes()).hexdigest()
+               for p in sorted(Path(d, "out").iterdir())}
+    (HERE / "ref_outputs.json").write_text(json.dumps(rec, indent=1))
+
+
 if __name__ == "__main__":
     R = po.RefLib()
-    params(R)
-    small(R)
-    config1(R)
-    if "--app" in sys.argv:
-        app()
+    if "--outputs-only" not in sys.argv:
+        params(R)
+        small(R)
+        config1(R)
+        if "--app" in sys.argv:
+            app()
+    outputs()
     print("golden fixtures written to", HERE)

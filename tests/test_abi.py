@@ -84,3 +84,22 @@ def test_fortran_shim_built():
     for mod in ("fvm_bardow", "periodic_lbm", "collision_bgk", "collision_trt",
                 "collision_regularized", "taylor_green", "precision"):
         assert (fb / "mod" / f"{mod}.mod").exists(), mod
+
+
+def test_sim_plugin_protocol_exported(P):
+    """c_plbm_{init,step,vars,free,norm} (include/plbm_sim.h) for the reference's ctypes
+    plugin loader; norm needs no GPU."""
+    import ctypes
+    text = (ROOT / "include" / "plbm_sim.h").read_text()
+    names = sorted(set(re.findall(r"\b(c_plbm_[a-z]+)\s*\(", text)))
+    assert names == ["c_plbm_free", "c_plbm_init", "c_plbm_norm", "c_plbm_step", "c_plbm_vars"]
+    plugin = P.sim.SimPlugin()
+    u = np.asfortranarray(np.arange(12.0).reshape(4, 3))
+    ua = np.asfortranarray(u + 0.5)
+    assert plugin.my_norm(u, ua) == pytest.approx(np.linalg.norm(u - ua) / np.linalg.norm(ua),
+                                                  rel=1e-15)
+    # dt != 1 is refused before any device work, as the reference plugin does
+    p0 = np.zeros((4, 3), order="F")
+    u0 = np.zeros((4, 3, 2), order="F")
+    with pytest.raises(RuntimeError):
+        plugin.init((4, 3), 0.5, p0, u0)

@@ -245,3 +245,25 @@ def test_autotune_keeps_state_and_results(P, gpu, oracle):
     np.testing.assert_array_equal(lat.download_f(iold)[:ny], latest[:ny])
     with pytest.raises(P.PlbmError):
         lat.set_launch("trt", 96, 0)
+
+
+@pytest.mark.parametrize("n,model,steps", [(4096, "trt", 20), (8192, "rr", 3), (8192, "trt", 3)])
+def test_full_size_bit_exact(P, gpu, oracle, n, model, steps):
+    """The benchmark sizes themselves: Taylor-Green + U(+-1e-3) noise (seed 1 at 4096^2 as
+    the 4096^2 TRT roofline case, seed 7 at 8192^2 as the headline case), FP64, a few
+    steps; GPU buffers equal the C oracle's (OpenMP on the host cores) bit for bit."""
+    import os
+    print(f"case full size {n} {model} {steps} steps", flush=True)
+    nu, u0, seed = (0.005, 0.04, 1) if n == 4096 else (0.01, 0.05, 7)
+    rho, ux, uy = tgv_fields(P, n, u0, nu)
+    P.add_noise(ux, uy, seed, 1e-3)
+    omega = 1.0 / (3 * nu + 0.5)
+    lat = P.Lattice(n, n, 2, "f64")
+    lat.set_params(omega, 0.25)
+    lat.equilibrium_init(0, rho, ux, uy)
+    iold, _ = lat.run(model, steps, 0, 1)
+    got = lat.download_f(iold)
+    lat.close()
+    latest, prev = oracle_run(oracle, model, n, n, rho, ux, uy, steps, omega)
+    del prev
+    assert np.array_equal(got, latest), "GPU and oracle populations differ"
