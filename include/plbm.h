@@ -128,6 +128,16 @@ int plbm_collide(plbm_lattice *h, int model, int buf);
  * src/fvm_bardow.F90:328). */
 int plbm_copy(plbm_lattice *h, int src, int dst);
 
+/* Streaming operators for the grid%streaming slot, src -> dst with time step dt:
+ * PLBM_STREAM_EXACT = lbm_stream (dt unused); the others are the reference's
+ * finite-volume / finite-difference schemes stream_fvm_bardow (src/fvm_bardow.F90:393-509),
+ * stream_fdm_bardow (:511-685, default stencil) and stream_fdm_sofonea (:688-893). */
+#define PLBM_STREAM_EXACT 0
+#define PLBM_STREAM_FVM_BARDOW 1
+#define PLBM_STREAM_FDM_BARDOW 2
+#define PLBM_STREAM_FDM_SOFONEA 3
+int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
+
 /* ---------------------------------------------------------------- read-back */
 
 /* update_macros (src/fvm_bardow.F90:343-390) on buffer buf: rho and u = j/rho, in the
@@ -154,6 +164,12 @@ int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, d
 /* shear_layer_t%eval (src/benchmarks/shear_layer.f90:123-150). */
 int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
                            double *rho, double *ux, double *uy);
+/* vortex_case_t%eval (src/benchmarks/barotropic_vortex_case.F90:34-83): convected
+ * barotropic vortex (advection u0, centre (xc, yc), radius rc, strength eps), rho0 = 1,
+ * csqr = 1/3. */
+int plbm_field_barotropic_vortex(int nx, int ny, double u0, double xc, double yc, double rc,
+                                 double eps, double rho0, double csqr, double *rho, double *ux,
+                                 double *uy);
 /* ux += amp*(2U-1), uy += amp*(2U'-1) with U, U' the counter-based deviates of cell c
  * (counters 2c and 2c+1) of `seed`; the generator is splitmix64's output function. */
 int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy);

@@ -92,6 +92,105 @@ void orc_stream(int nx, int ny, int ld, const double *fsrc, double *fdst) {
     }
 }
 
+/* ------------------------------------------------------------------ non-LBM streaming */
+
+/* The reference's finite-volume / finite-difference streaming operators.  They loop
+ * x = 1..ny, y = 1..nx (src/fvm_bardow.F90:434,437 and 724-726 ...), i.e. square grids;
+ * here x runs over nx and y over ny, identical on square grids.  Rest population copied. */
+#define IDX(q, x, y) (((size_t)(q) * (size_t)nx + (size_t)(x)) * (size_t)ld + (size_t)(y))
+
+/* stream_fvm_bardow, src/fvm_bardow.F90:410-509 (bilinear face interpolation). */
+void orc_stream_fvm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt) {
+    const double p2 = 0.5, p8 = 0.125;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) fnew[IDX(0, x, y)] = fold[IDX(0, x, y)];
+    for (int q = 1; q < 9; ++q) {
+        const double cxq = dt * ORC_CX[q], cyq = dt * ORC_CY[q];
+        #pragma omp parallel for schedule(static)
+        for (int x = 0; x < nx; ++x) {
+            const int xp = (x + 1) % nx, xm = (x + nx - 1) % nx;
+            for (int y = 0; y < ny; ++y) {
+                const int yp = (y + 1) % ny, ym = (y + ny - 1) % ny;
+                const double fc = fold[IDX(q, x, y)], fe = fold[IDX(q, xp, y)],
+                             fn = fold[IDX(q, x, yp)], fw = fold[IDX(q, xm, y)],
+                             fs = fold[IDX(q, x, ym)], fne = fold[IDX(q, xp, yp)],
+                             fnw = fold[IDX(q, xm, yp)], fsw = fold[IDX(q, xm, ym)],
+                             fse = fold[IDX(q, xp, ym)];
+                const double cfw = (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) -
+                                   (p8 * cyq) * (((fnw + fn) - fsw) - fs);
+                const double cfn = (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) -
+                                   (p8 * cxq) * (((fne + fe) - fnw) - fw);
+                const double cfe = (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) -
+                                   (p8 * cyq) * (((fne + fn) - fse) - fs);
+                const double cfs = (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) -
+                                   (p8 * cxq) * (((fse + fe) - fsw) - fw);
+                fnew[IDX(q, x, y)] = (fc - cxq * (cfe - cfw)) - cyq * (cfn - cfs);
+            }
+        }
+    }
+}
+
+/* stream_fdm_bardow, src/fvm_bardow.F90:526-685, default stencil (no FDM_* option):
+ * central first and second differences, Lax-Wendroff update. */
+void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt) {
+    const double p2 = 0.5;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) fnew[IDX(0, x, y)] = fold[IDX(0, x, y)];
+    for (int q = 1; q < 9; ++q) {
+        const double cxq = dt * ORC_CX[q], cyq = dt * ORC_CY[q];
+        const double cxxq = (0.5 * cxq) * cxq, cyyq = (0.5 * cyq) * cyq, cxyq = cxq * cyq;
+        #pragma omp parallel for schedule(static)
+        for (int x = 0; x < nx; ++x) {
+            const int xp = (x + 1) % nx, xm = (x + nx - 1) % nx;
+            for (int y = 0; y < ny; ++y) {
+                const int yp = (y + 1) % ny, ym = (y + ny - 1) % ny;
+                const double fc = fold[IDX(q, x, y)], fe = fold[IDX(q, xp, y)],
+                             fn = fold[IDX(q, x, yp)], fw = fold[IDX(q, xm, y)],
+                             fs = fold[IDX(q, x, ym)], fne = fold[IDX(q, xp, yp)],
+                             fnw = fold[IDX(q, xm, yp)], fsw = fold[IDX(q, xm, ym)],
+                             fse = fold[IDX(q, xp, ym)];
+                const double dfx = p2 * (fe - fw), dfy = p2 * (fn - fs);
+                const double dfxx = (fe - 2.0 * fc) + fw, dfyy = (fn - 2.0 * fc) + fs;
+                const double dfxy = 0.25 * (((fne - fse) - fnw) + fsw);
+                fnew[IDX(q, x, y)] =
+                    ((fc - cxq * dfx) - cyq * dfy) + ((cxxq * dfxx + cxyq * dfxy) + cyyq * dfyy);
+            }
+        }
+    }
+}
+
+/* stream_fdm_sofonea, src/fvm_bardow.F90:702-889: per direction, a 1-D Lax-Wendroff
+ * update along c_q between the upstream node fu and the downstream node fd. */
+void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *fnew, double dt) {
+    const double p2 = 0.5 / sqrt(2.0);
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) fnew[IDX(0, x, y)] = fold[IDX(0, x, y)];
+    /* (dx, dy) of fu for q = 1..8; fd is the opposite neighbour */
+    static const int UX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static const int UY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    for (int q = 1; q < 9; ++q) {
+        #pragma omp parallel for schedule(static)
+        for (int x = 0; x < nx; ++x) {
+            const int xu = (x + UX[q] + nx) % nx, xd = (x - UX[q] + nx) % nx;
+            for (int y = 0; y < ny; ++y) {
+                const int yu = (y + UY[q] + ny) % ny, yd = (y - UY[q] + ny) % ny;
+                const double fc = fold[IDX(q, x, y)], fu = fold[IDX(q, xu, yu)],
+                             fd = fold[IDX(q, xd, yd)];
+                double du1, du2;
+                if (q < 5) {
+                    du1 = 0.5 * (fu - fd);
+                    du2 = (fu - 2.0 * fc) + fd;
+                } else {
+                    du1 = p2 * (fu - fd);
+                    du2 = 0.5 * ((fu - 2.0 * fc) + fd);
+                }
+                fnew[IDX(q, x, y)] = fc + dt * ((0.5 * dt) * du2 - du1);
+            }
+        }
+    }
+}
+#undef IDX
+
 /* ------------------------------------------------------------------ collisions */
 
 /* src/collision_bgk.F90:35-82 (bgk_kernel; the default build, no -DSPLIT).

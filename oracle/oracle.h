@@ -34,6 +34,10 @@ double orc_lambda_d(double omega, double x);
 int orc_padded_ld(int ny);
 
 void orc_stream(int nx, int ny, int ld, const double *fsrc, double *fdst);
+/* Finite-volume / finite-difference streaming operators (src/fvm_bardow.F90:393-893) */
+void orc_stream_fvm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
+void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
+void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
 void orc_collide_bgk(int nx, int ny, int ld, double *f, double omega);
 void orc_collide_trt(int nx, int ny, int ld, double *f, double lambda_e, double lambda_d);
 void orc_collide_rr(int nx, int ny, int ld, double *f, double omega);

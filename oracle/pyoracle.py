@@ -71,6 +71,8 @@ class Oracle:
         L.orc_padded_ld.argtypes = [_i]
         L.orc_padded_ld.restype = _i
         L.orc_stream.argtypes = [_i, _i, _i, _dp, _dp]
+        for name in ("orc_stream_fvm_bardow", "orc_stream_fdm_bardow", "orc_stream_fdm_sofonea"):
+            getattr(L, name).argtypes = [_i, _i, _i, _dp, _dp, _d]
         L.orc_collide_bgk.argtypes = [_i, _i, _i, _dp, _d]
         L.orc_collide_trt.argtypes = [_i, _i, _i, _dp, _d, _d]
         L.orc_collide_rr.argtypes = [_i, _i, _i, _dp, _d]
@@ -112,6 +114,12 @@ class Oracle:
     # array kernels (in place where the reference is in place)
     def stream(self, nx, ny, fsrc, fdst):
         self.lib.orc_stream(nx, ny, fsrc.shape[0], _ptr(fsrc), _ptr(fdst))
+
+    def stream_op(self, op, nx, ny, fsrc, fdst, dt=1.0):
+        """op: "exact" | "fvm_bardow" | "fdm_bardow" | "fdm_sofonea"."""
+        if op == "exact":
+            return self.stream(nx, ny, fsrc, fdst)
+        getattr(self.lib, "orc_stream_" + op)(nx, ny, fsrc.shape[0], _ptr(fsrc), _ptr(fdst), dt)
 
     def collide(self, model, nx, ny, f, omega, trt_magic=0.25):
         self.lib.orc_collide(MODELS.get(model, model), nx, ny, f.shape[0], _ptr(f), omega,
@@ -190,6 +198,10 @@ class RefLib:
         L.ref_lambda_d.restype = _d
         L.ref_tgv_eval.argtypes = [_i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp]
         L.ref_shear_eval.argtypes = [_i, _i, _d, _d, _d, _d, _dp, _dp, _dp]
+        L.ref_vortex_eval.argtypes = [_i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp]
+        L.ref_set_streaming.argtypes = [ctypes.c_void_p, _i]
+        L.ref_streaming.argtypes = [ctypes.c_void_p]
+        L.ref_perform_step.argtypes = [ctypes.c_void_p, _i]
 
     def equilibrium(self, rho, ux, uy):
         out = np.zeros(9)
@@ -210,6 +222,11 @@ class RefLib:
     def shear(self, nx, ny, length, u0, k, delta):
         rho, ux, uy = new_field(nx, ny), new_field(nx, ny), new_field(nx, ny)
         self.lib.ref_shear_eval(nx, ny, length, u0, k, delta, _ptr(rho), _ptr(ux), _ptr(uy))
+        return rho, ux, uy
+
+    def vortex(self, nx, ny, u0, xc, yc, rc, eps):
+        rho, ux, uy = new_field(nx, ny), new_field(nx, ny), new_field(nx, ny)
+        self.lib.ref_vortex_eval(nx, ny, u0, xc, yc, rc, eps, _ptr(rho), _ptr(ux), _ptr(uy))
         return rho, ux, uy
 
     def grid(self, nx, ny, model, nf=2):
@@ -264,6 +281,17 @@ class RefGrid:
 
     def stream(self):
         self.ref.lib.ref_stream(self.h)
+
+    STREAMING = {"exact": 0, "fvm_bardow": 1, "fdm_bardow": 2, "fdm_sofonea": 3}
+
+    def set_streaming(self, op):
+        self.ref.lib.ref_set_streaming(self.h, self.STREAMING.get(op, op))
+
+    def streaming(self):
+        self.ref.lib.ref_streaming(self.h)
+
+    def perform_step(self, n=1):
+        self.ref.lib.ref_perform_step(self.h, n)
 
     def collide(self):
         self.ref.lib.ref_collide(self.h)

@@ -23,6 +23,7 @@ module ref_driver
    use collision_regularized, only: collide_rr
    use taylor_green, only: taylor_green_t
    use shear_layer_case, only: shear_layer_t
+   use barotropic_vortex_case, only: vortex_case_t
 
    implicit none
 
@@ -164,6 +165,44 @@ contains
       call lbm_stream(p%grid)
    end subroutine
 
+   subroutine ref_set_streaming(h, op) bind(c, name="ref_set_streaming")
+      ! 0 lbm_stream, 1 stream_fvm_bardow, 2 stream_fdm_bardow, 3 stream_fdm_sofonea
+      type(c_ptr), value :: h
+      integer(c_int), value :: op
+      type(grid_box), pointer :: p
+      p => unbox(h)
+      select case (op)
+      case (0)
+         p%grid%streaming => lbm_stream
+      case (1)
+         p%grid%streaming => stream_fvm_bardow
+      case (2)
+         p%grid%streaming => stream_fdm_bardow
+      case (3)
+         p%grid%streaming => stream_fdm_sofonea
+      case default
+         error stop "ref_driver: unknown streaming operator"
+      end select
+   end subroutine
+
+   subroutine ref_streaming(h) bind(c, name="ref_streaming")
+      type(c_ptr), value :: h
+      type(grid_box), pointer :: p
+      p => unbox(h)
+      call p%grid%streaming()
+   end subroutine
+
+   subroutine ref_perform_step(h, nsteps) bind(c, name="ref_perform_step")
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+      type(grid_box), pointer :: p
+      integer :: s
+      p => unbox(h)
+      do s = 1, nsteps
+         call perform_step(p%grid)
+      end do
+   end subroutine
+
    subroutine ref_collide(h) bind(c, name="ref_collide")
       type(c_ptr), value :: h
       type(grid_box), pointer :: p
@@ -235,6 +274,15 @@ contains
       type(taylor_green_t) :: tg
       tg = taylor_green_t(nx, ny, kx, ky, umax, nu)
       call tg%eval(t, p, ux, uy)
+   end subroutine
+
+   subroutine ref_vortex_eval(nx, ny, u0, xc, yc, rc, eps, rho, ux, uy) bind(c, name="ref_vortex_eval")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: u0, xc, yc, rc, eps
+      real(c_double), intent(out) :: rho(ny,nx), ux(ny,nx), uy(ny,nx)
+      type(vortex_case_t) :: vc
+      vc = vortex_case_t(u0, xc, yc, rc, eps=eps)
+      call vc%eval(nx, ny, rho, ux, uy)
    end subroutine
 
    subroutine ref_shear_eval(nx, ny, length, u0, k, delta, rho, ux, uy) bind(c, name="ref_shear_eval")

@@ -23,7 +23,12 @@ __all__ = [
     "set_pdf_to_equilibrium", "set_properties", "sync_device", "sync_host", "update_macros",
     "w",
 ]
+from .lattice import (  # noqa: E402
+    VortexCase, stream_fdm_bardow, stream_fdm_sofonea, stream_fvm_bardow,
+)
 from .batch import Batch, Case, sweep_cases  # noqa: E402
+
+__all__ += ["VortexCase", "stream_fdm_bardow", "stream_fdm_sofonea", "stream_fvm_bardow"]
 
 __all__ += ["Batch", "Case", "sweep_cases"]
 from .adjoint import Adjoint, band_limited_field, velocity_misfit  # noqa: E402

@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip"]
+HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip"]
 CXX_SOURCES = ["fields.cpp", "sim_plugin.cpp"]
 # -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
 # reference is built without FMA); kernels that want FMA enable it locally by pragma.
@@ -45,7 +45,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-Wa
 FORTRAN_SOURCES = [
     "plbm_capi.f90", "precision.F90", "fvm_bardow.F90", "periodic_lbm.f90",
     "collision_bgk.f90", "collision_trt.f90", "collision_regularized.f90",
-    "taylor_green.f90", "shear_layer.f90",
+    "taylor_green.f90", "shear_layer.f90", "barotropic_vortex_case.f90",
 ]
 
 
@@ -119,14 +119,15 @@ def build_fortran_tests(verbose: bool = True) -> None:
                 print(f"[build] {(out / name).relative_to(ROOT)}")
 
 
-def build_app(ref_root: Path, verbose: bool = True) -> Path | None:
-    src = ref_root / "app" / "main_taylor_green.f90"
-    if not src.exists():
-        return None
-    out = link_fortran_program(src, APPDIR / "main_taylor_green")
-    if verbose:
-        print(f"[build] {out.relative_to(ROOT)} (unchanged {src.name} on the HIP engine)")
-    return out
+def build_app(ref_root: Path, verbose: bool = True) -> None:
+    """The reference's driver programs, compiled unmodified against the shim."""
+    for name in ("main_taylor_green", "main_vortex"):
+        src = ref_root / "app" / f"{name}.f90"
+        if not src.exists():
+            continue
+        out = link_fortran_program(src, APPDIR / name, moddir=APPDIR / f"mod_{name}")
+        if verbose:
+            print(f"[build] {out.relative_to(ROOT)} (unchanged {src.name} on the HIP engine)")
 
 
 def build_oracle(ref_root: Path | None, verbose: bool = True) -> None:

@@ -59,6 +59,32 @@ int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, d
     return PLBM_OK;
 }
 
+// eval_vortex_case, src/benchmarks/barotropic_vortex_case.F90:34-83 (Wissocq et al. 2020
+// convected vortex with barotropic density, Eqs. (3), (4), (20)).
+int plbm_field_barotropic_vortex(int nx, int ny, double u0, double xc, double yc, double rc,
+                                 double eps, double rho0, double csqr, double *rho, double *ux,
+                                 double *uy) {
+    if (nx < 1 || ny < 1 || !rho || !ux || !uy || !(rc > 0) || !(csqr > 0)) return PLBM_ERR_ARG;
+    const double half = 1.0 / 2.0;
+    const double rcsqr = rc * rc;
+    const double vma_sq = (eps * eps) / csqr;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        const double xl = double(x) + 0.5;
+        for (int y = 0; y < ny; ++y) {
+            const double yl = double(y) + 0.5;
+            const double xr = xl - xc, yr = yl - yc;
+            const double rsqr = xr * xr + yr * yr;
+            const double g = std::exp(-((half * rsqr) / rcsqr));
+            const size_t c = (size_t)x * ny + y;
+            ux[c] = u0 - (eps * (yr / rc)) * g;
+            uy[c] = (eps * (xr / rc)) * g;
+            rho[c] = rho0 * std::exp(-((half * vma_sq) * std::exp(-(rsqr / rcsqr))));
+        }
+    }
+    return PLBM_OK;
+}
+
 // Counter-based uniform deviate in [0,1): splitmix64's output function applied to
 // seed + (counter+1)*0x9E3779B97F4A7C15, top 53 bits.
 double plbm_noise(uint64_t seed, uint64_t counter) {

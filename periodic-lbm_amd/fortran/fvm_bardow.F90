@@ -353,7 +353,10 @@ contains
 
    logical function is_engine_op_s(grid) result(yes)
       type(lattice_grid), intent(in) :: grid
-      yes = associated(grid%streaming, plbm_lbm_stream)
+      yes = associated(grid%streaming, plbm_lbm_stream) .or. &
+            associated(grid%streaming, stream_fvm_bardow) .or. &
+            associated(grid%streaming, stream_fdm_bardow) .or. &
+            associated(grid%streaming, stream_fdm_sofonea)
    end function
 
    logical function is_engine_op_c(grid) result(yes)
@@ -473,21 +476,35 @@ contains
 
    ! ------------------------------------------------------------------ non-LBM streaming
 
-   !> The finite-difference / finite-volume streaming operators of the reference
-   !> (src/fvm_bardow.F90:393-893) are not part of this engine.
+   subroutine stream_op_on_device(grid, op, name)
+      class(lattice_grid), intent(inout) :: grid
+      integer(c_int), intent(in) :: op
+      character(len=*), intent(in) :: name
+      call require_device(grid)
+      call plbm_check(plbm_stream_op(grid%dev, op, int(grid%iold-1,c_int), &
+         int(grid%inew-1,c_int), grid%dt), name)
+      call device_written(grid)
+   end subroutine
+
+   !> stream_fvm_bardow (src/fvm_bardow.F90:393-509): finite-volume streaming with
+   !> bilinear face interpolation, f(iold) -> f(inew), time step grid%dt, on the device.
    subroutine stream_fvm_bardow(grid)
       class(lattice_grid), intent(inout) :: grid
-      error stop "stream_fvm_bardow: not provided by the GPU engine (exact streaming only)"
+      call stream_op_on_device(grid, PLBM_STREAM_FVM_BARDOW, "stream_fvm_bardow")
    end subroutine
 
+   !> stream_fdm_bardow (src/fvm_bardow.F90:511-685, default stencil): Lax-Wendroff
+   !> finite-difference streaming on the device.
    subroutine stream_fdm_bardow(grid)
       class(lattice_grid), intent(inout) :: grid
-      error stop "stream_fdm_bardow: not provided by the GPU engine (exact streaming only)"
+      call stream_op_on_device(grid, PLBM_STREAM_FDM_BARDOW, "stream_fdm_bardow")
    end subroutine
 
+   !> stream_fdm_sofonea (src/fvm_bardow.F90:688-893): 1-D Lax-Wendroff along each
+   !> lattice direction on the device.
    subroutine stream_fdm_sofonea(grid)
       class(lattice_grid), intent(inout) :: grid
-      error stop "stream_fdm_sofonea: not provided by the GPU engine (exact streaming only)"
+      call stream_op_on_device(grid, PLBM_STREAM_FDM_SOFONEA, "stream_fdm_sofonea")
    end subroutine
 
    ! ------------------------------------------------------------------ output

@@ -14,6 +14,9 @@ module plbm_capi
 
    public :: plbm_create, plbm_destroy, plbm_set_params, plbm_upload_f, plbm_download_f
    public :: plbm_equilibrium_init, plbm_step, plbm_run, plbm_stream, plbm_collide, plbm_copy
+   public :: plbm_stream_op
+   integer(c_int), parameter, public :: PLBM_STREAM_EXACT = 0, PLBM_STREAM_FVM_BARDOW = 1, &
+                                        PLBM_STREAM_FDM_BARDOW = 2, PLBM_STREAM_FDM_SOFONEA = 3
    public :: plbm_macros, plbm_reduce, plbm_sync, plbm_last_error, plbm_check
    public :: plbm_lambda_d, plbm_magic_number
 
@@ -74,6 +77,13 @@ module plbm_capi
          import :: c_int, c_ptr
          type(c_ptr), value :: h
          integer(c_int), value :: src, dst
+      end function
+
+      integer(c_int) function plbm_stream_op(h, op, src, dst, dt) bind(c, name="plbm_stream_op")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: op, src, dst
+         real(c_double), value :: dt
       end function
 
       integer(c_int) function plbm_collide(h, model, buf) bind(c, name="plbm_collide")

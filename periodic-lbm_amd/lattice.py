@@ -135,6 +135,10 @@ class Lattice:
     def stream(self, src: int, dst: int):
         check(lib().plbm_stream(self.h, src, dst))
 
+    def stream_op(self, op: str | int, src: int, dst: int, dt: float = 1.0):
+        """op: "exact" | "fvm_bardow" | "fdm_bardow" | "fdm_sofonea"."""
+        check(lib().plbm_stream_op(self.h, _abi.STREAM_OPS.get(op, op), src, dst, dt))
+
     def collide(self, model: str | int, buf: int):
         check(lib().plbm_collide(self.h, _abi.MODEL_IDS.get(model, model), buf))
 
@@ -300,6 +304,27 @@ def lbm_stream(grid: LatticeGrid) -> None:
     _device_written(grid)
 
 
+def _stream_op(grid: LatticeGrid, op: str):
+    _require_device(grid)
+    grid.lattice.stream_op(op, grid.iold - 1, grid.inew - 1, grid.dt)
+    _device_written(grid)
+
+
+def stream_fvm_bardow(grid: LatticeGrid) -> None:
+    """stream_fvm_bardow (src/fvm_bardow.F90:393-509) on the device, time step grid.dt."""
+    _stream_op(grid, "fvm_bardow")
+
+
+def stream_fdm_bardow(grid: LatticeGrid) -> None:
+    """stream_fdm_bardow (src/fvm_bardow.F90:511-685, default stencil) on the device."""
+    _stream_op(grid, "fdm_bardow")
+
+
+def stream_fdm_sofonea(grid: LatticeGrid) -> None:
+    """stream_fdm_sofonea (src/fvm_bardow.F90:688-893) on the device."""
+    _stream_op(grid, "fdm_sofonea")
+
+
 def _collide(grid: LatticeGrid, model: int):
     _require_device(grid)
     grid.lattice.collide(model, grid.inew - 1)
@@ -322,6 +347,7 @@ def collide_rr(grid: LatticeGrid) -> None:
 
 
 _ENGINE_COLLISIONS = {collide_bgk: _abi.BGK, collide_trt: _abi.TRT, collide_rr: _abi.RR}
+_ENGINE_STREAMING = (lbm_stream, stream_fvm_bardow, stream_fdm_bardow, stream_fdm_sofonea)
 
 
 def _dispatch(grid: LatticeGrid, op, engine_ops):
@@ -339,7 +365,7 @@ def perform_step(grid: LatticeGrid) -> None:
         grid.lattice.step(model, grid.iold - 1, grid.inew - 1)
         _device_written(grid)
     else:
-        _dispatch(grid, grid.streaming, (lbm_stream,))
+        _dispatch(grid, grid.streaming, _ENGINE_STREAMING)
         _dispatch(grid, grid.collision, _ENGINE_COLLISIONS)
     grid.iold, grid.inew = grid.inew, grid.iold
 
@@ -349,7 +375,7 @@ perform_lbm_step = perform_step
 
 def perform_triple_step(grid: LatticeGrid) -> None:
     """perform_triple_step (src/fvm_bardow.F90:322-340)."""
-    _dispatch(grid, grid.streaming, (lbm_stream,))
+    _dispatch(grid, grid.streaming, _ENGINE_STREAMING)
     if grid._host_current:
         grid._f[:, :, :, grid.iold - 1] = grid._f[:, :, :, grid.inew - 1]
     else:
@@ -420,6 +446,21 @@ class TaylorGreen:
         check(lib().plbm_field_taylor_green(self.nx, self.ny, self.kx, self.ky, self.umax,
                                             self.nu, t, dptr(p), dptr(ux), dptr(uy)))
         return p, ux, uy
+
+
+class VortexCase:
+    """vortex_case_t (src/benchmarks/barotropic_vortex_case.F90:20-83)."""
+
+    def __init__(self, U0, xc, yc, Rc, eps, rho0=1.0, csqr_=1.0 / 3.0):
+        self.U0, self.xc, self.yc, self.Rc, self.eps = U0, xc, yc, Rc, eps
+        self.rho0, self.csqr = rho0, csqr_
+
+    def eval(self, nx: int, ny: int):
+        rho, ux, uy = (np.zeros((ny, nx), order="F") for _ in range(3))
+        check(lib().plbm_field_barotropic_vortex(nx, ny, self.U0, self.xc, self.yc, self.Rc,
+                                                 self.eps, self.rho0, self.csqr, dptr(rho),
+                                                 dptr(ux), dptr(uy)))
+        return rho, ux, uy
 
 
 class ShearLayer:

@@ -140,6 +140,40 @@ def app():
                                                    "values": vals}, indent=1))
 
 
+def streaming_ops(R):
+    """perform_step with the finite-volume / finite-difference streaming operators and
+    BGK on a 16x16 Taylor-Green start, dt = 1 and 0.5: populations after 5 steps."""
+    out = {}
+    n = 16
+    rho, ux, uy = tgv_init(R, n, n, 0.05, 0.02)
+    for op in ("fvm_bardow", "fdm_bardow", "fdm_sofonea"):
+        for dt in (1.0, 0.5):
+            g = R.grid(n, n, "bgk")
+            g.set_properties(0.02, dt, 0.25)
+            g.set_streaming(op)
+            g.set_macros(rho, ux, uy)
+            g.set_pdf_to_equilibrium()
+            g.perform_step(5)
+            iold, _, _, _ = g.indices()
+            out[f"{op}_dt{dt}"] = g.get_f(iold)[:n]
+            out[f"{op}_dt{dt}_omega"] = np.array([g.params()["omega"]])
+            g.close()
+    out["rho"], out["ux"], out["uy"] = rho, ux, uy
+    np.savez_compressed(HERE / "ref_streaming_ops.npz", **out)
+
+
+def vortex_app():
+    """The unchanged app/main_vortex.f90 (128^2, stream_fvm_bardow + collide_bgk) run as
+    `main_vortex 1.0`: its printout and the sha256 of the files it writes."""
+    exe = po.HERE / "_ref" / "main_vortex_ref"
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([str(exe), "1.0"], cwd=d, capture_output=True, text=True, check=True)
+        files = {p.name: hashlib.sha256(p.read_bytes()).hexdigest()
+                 for p in sorted(Path(d, "vortex").iterdir())}
+    (HERE / "ref_vortex_app.json").write_text(json.dumps(
+        {"args": ["1.0"], "stdout": r.stdout, "files": files}, indent=1))
+
+
 def outputs():
     """sha256 of the files tests/fortran/outputs_check.f90 writes through the reference's
     output_vtk / output_npy / output_gnuplot (oracle/_ref/outputs_check_ref)."""
@@ -153,11 +187,17 @@ def outputs():
 
 if __name__ == "__main__":
     R = po.RefLib()
+    if "--streaming-only" in sys.argv:
+        streaming_ops(R)
+        vortex_app()
+        sys.exit(0)
     if "--outputs-only" not in sys.argv:
         params(R)
         small(R)
         config1(R)
+        streaming_ops(R)
         if "--app" in sys.argv:
             app()
+            vortex_app()
     outputs()
     print("golden fixtures written to", HERE)

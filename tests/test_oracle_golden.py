@@ -84,3 +84,22 @@ def test_reference_app_record():
     A = json.loads((GOLDEN / "ref_app.json").read_text())
     assert A["values"]["L2-norm"] == pytest.approx(3.7266899791616004e-05, rel=0, abs=0)
     assert A["values"]["Final time"] == 48658.0
+
+
+@pytest.mark.parametrize("op", ["fvm_bardow", "fdm_bardow", "fdm_sofonea"])
+@pytest.mark.parametrize("dt", [1.0, 0.5])
+def test_streaming_operators_bitwise(oracle, op, dt):
+    """perform_step with stream_fvm_bardow / stream_fdm_bardow / stream_fdm_sofonea and
+    collide_bgk, 5 steps on 16x16 (reference fixtures)."""
+    G = np.load(GOLDEN / "ref_streaming_ops.npz")
+    n = 16
+    rho, ux, uy = (np.asfortranarray(G[k]) for k in ("rho", "ux", "uy"))
+    omega = float(G[f"{op}_dt{dt}_omega"][0])
+    assert omega == oracle.set_properties(0.02, dt, 0.25)["omega"]
+    a, b = po.new_f(n, n), po.new_f(n, n)
+    oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, a)
+    for _ in range(5):
+        oracle.stream_op(op, n, n, a, b, dt)
+        oracle.collide("bgk", n, n, b, omega)
+        a, b = b, a
+    np.testing.assert_array_equal(a[:n], G[f"{op}_dt{dt}"])

@@ -278,3 +278,38 @@ def test_rr_nonequilibrium_stress_relaxes_by_one_minus_omega(oracle):
             pre = moments(f0[y, x] - feq)[3:]
             post = moments(f[y, x] - feq)[3:]
             np.testing.assert_allclose(post, (1 - om) * np.array(pre), rtol=1e-10, atol=1e-17)
+
+
+@pytest.mark.parametrize("op", ["fvm_bardow", "fdm_bardow", "fdm_sofonea"])
+def test_fd_streaming_is_lax_wendroff_along_axes(oracle, op):
+    """Along an axis direction all three schemes reduce to 1-D Lax-Wendroff:
+    e^{ikx} -> G e^{ikx}, G = 1 - i c sin k - c^2 (1 - cos k), c = dt (closed form)."""
+    nx, ny = 32, 8
+    kx = 2 * np.pi * 3 / nx
+    x = np.arange(nx)
+    for dt in (1.0, 0.6):
+        f = po.new_f(nx, ny)
+        f[:ny, :, 1] = np.cos(kx * x)[None, :]  # q = 1: c = (1, 0)
+        out = po.new_f(nx, ny)
+        oracle.stream_op(op, nx, ny, f, out, dt)
+        G = 1 - 1j * dt * np.sin(kx) - dt * dt * (1 - np.cos(kx))
+        want = np.real(G * np.exp(1j * kx * x))
+        np.testing.assert_allclose(out[:ny, :, 1], np.broadcast_to(want, (ny, nx)), atol=1e-14)
+
+
+@pytest.mark.parametrize("op", ["fvm_bardow", "fdm_bardow", "fdm_sofonea"])
+def test_fd_streaming_conserves_and_keeps_constants(oracle, op):
+    nx, ny = 9, 7
+    rng = np.random.default_rng(4)
+    f = po.new_f(nx, ny)
+    f[:ny] = rng.random((ny, nx, 9))
+    out = po.new_f(nx, ny)
+    oracle.stream_op(op, nx, ny, f, out, 0.8)
+    np.testing.assert_allclose(out[:ny].sum(axis=(0, 1)), f[:ny].sum(axis=(0, 1)), rtol=1e-13)
+    np.testing.assert_array_equal(out[:ny, :, 0], f[:ny, :, 0])  # rest population copied
+    c = po.new_f(nx, ny)
+    c[:ny] = 0.37
+    oracle.stream_op(op, nx, ny, c, out, 0.8)
+    np.testing.assert_allclose(out[:ny], 0.37, rtol=1e-15)
+    oracle.stream_op(op, nx, ny, f, out, 0.0)  # dt = 0: identity
+    np.testing.assert_array_equal(out[:ny], f[:ny])
