@@ -138,6 +138,26 @@ int plbm_copy(plbm_lattice *h, int src, int dst);
 #define PLBM_STREAM_FDM_SOFONEA 3
 int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
 
+/* DUGKS, src/periodic_dugks.F90 (default build, without -DDUGKS), FP64 storage only
+ * (PLBM_ERR_STATE otherwise).  omega_full is grid%omega, omega_half the reference's
+ * 1/(tau/dt + 1/2) (:56).
+ *   plbm_dugks_collide  dugks_collide (:46-77): dst = BGK(src, omega_full),
+ *                       src = BGK(src, omega_half) in place.
+ *   plbm_dugks_stream   dugks_stream / kernel_stream (:168-299): dst_q -= dt c_q . (face
+ *                       flux balance of src), q = 1..8.
+ *   plbm_dugks_step     nsteps x perform_dugks_step (:25-38) on slots (iold, inew) as they
+ *                       are on entry.  On return every slot holds what the reference's
+ *                       buffer of the same index holds after the same steps (the caller
+ *                       swaps its iold/inew ids once per step, as the reference does).
+ *                       The fused form moves 360 bytes per cell per step; when nsteps is
+ *                       odd the two slots exchange their device storage, so addresses from
+ *                       plbm_buffer_ptr for them must be fetched again.  Asynchronous. */
+int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omega_full,
+                       double omega_half);
+int plbm_dugks_stream(plbm_lattice *h, int src, int dst, double dt);
+int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, double dt,
+                    double omega_full, double omega_half);
+
 /* ---------------------------------------------------------------- read-back */
 
 /* update_macros (src/fvm_bardow.F90:343-390) on buffer buf: rho and u = j/rho, in the

@@ -189,6 +189,90 @@ void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *
         }
     }
 }
+
+/* ------------------------------------------------------------------ DUGKS
+ * src/periodic_dugks.F90, default build (no -DDUGKS; src/CMakeLists.txt defines none).
+ * Pinned bit for bit against the flang-built reference (ref_dugks_* in ref_driver.f90);
+ * invariants: omega = 1 in the collision gives the SPLIT-form equilibrium, the stream
+ * conserves mass and momentum on a periodic grid, dt = 0 leaves fp unchanged. */
+
+/* kernel_bgk, src/periodic_dugks.F90:80-165: BGK in the SPLIT algebra on one cell. */
+static void dugks_bgk_cell(double f[9], double omega) {
+    const double omegabar = 1.0 - omega;
+    const double om_w0 = (3.0 * omega) * (4.0 / 9.0), om_ws = (3.0 * omega) * (1.0 / 9.0),
+                 om_wd = (3.0 * omega) * (1.0 / 36.0);
+    const double rho =
+        (((f[5] + f[7]) + (f[6] + f[8])) + ((f[1] + f[3]) + (f[2] + f[4]))) + f[0];
+    const double invrho = 1.0 / rho;
+    const double ux = invrho * (((f[5] - f[7]) + (f[8] - f[6])) + (f[1] - f[3]));
+    const double uy = invrho * (((f[5] - f[7]) + (f[6] - f[8])) + (f[2] - f[4]));
+    const double indp = 1.0 / 3.0 - 0.5 * (ux * ux + uy * uy);
+    f[0] = omegabar * f[0] + (om_w0 * rho) * indp;
+    const double t13 = indp + (1.5 * ux) * ux;
+    f[1] = omegabar * f[1] + (om_ws * rho) * (t13 + ux);
+    f[3] = omegabar * f[3] + (om_ws * rho) * (t13 - ux);
+    const double t24 = indp + (1.5 * uy) * uy;
+    f[2] = omegabar * f[2] + (om_ws * rho) * (t24 + uy);
+    f[4] = omegabar * f[4] + (om_ws * rho) * (t24 - uy);
+    const double xpy = ux + uy, t57 = indp + (1.5 * xpy) * xpy;
+    f[5] = omegabar * f[5] + (om_wd * rho) * (t57 + xpy);
+    f[7] = omegabar * f[7] + (om_wd * rho) * (t57 - xpy);
+    const double xmy = ux - uy, t68 = indp + (1.5 * xmy) * xmy;
+    f[6] = omegabar * f[6] + (om_wd * rho) * (t68 - xmy);
+    f[8] = omegabar * f[8] + (om_wd * rho) * (t68 + xmy);
+}
+
+/* dugks_collide, src/periodic_dugks.F90:46-77: fnew = BGK(fold, omega_full) (the copy
+ * then kernel_bgk on inew with grid%omega), fold = BGK(fold, omega_half) in place (the
+ * caller passes omega_half = 1/(tau/dt + 1/2)). */
+void orc_dugks_collide(int nx, int ny, int ld, double *fold, double *fnew, double omega_full,
+                       double omega_half) {
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        for (int y = 0; y < ny; ++y) {
+            double a[9], b[9];
+            for (int q = 0; q < 9; ++q) a[q] = b[q] = fold[IDX(q, x, y)];
+            dugks_bgk_cell(a, omega_full);
+            dugks_bgk_cell(b, omega_half);
+            for (int q = 0; q < 9; ++q) {
+                fnew[IDX(q, x, y)] = a[q];
+                fold[IDX(q, x, y)] = b[q];
+            }
+        }
+    }
+}
+
+/* dugks_stream / kernel_stream, src/periodic_dugks.F90:168-299 (default build): face
+ * values of ft by the bilinear interpolation of stream_fvm_bardow, then the flux balance
+ * subtracted from fp for q = 1..8; fp(q=0) is left as it is. */
+void orc_dugks_stream(int nx, int ny, int ld, const double *ft, double *fp, double dt) {
+    const double p2 = 0.5, p8 = 0.125;
+    for (int q = 1; q < 9; ++q) {
+        const double cxq = dt * ORC_CX[q], cyq = dt * ORC_CY[q];
+        #pragma omp parallel for schedule(static)
+        for (int x = 0; x < nx; ++x) {
+            const int xp = (x + 1) % nx, xm = (x + nx - 1) % nx;
+            for (int y = 0; y < ny; ++y) {
+                const int yp = (y + 1) % ny, ym = (y + ny - 1) % ny;
+                const double fc = ft[IDX(q, x, y)], fe = ft[IDX(q, xp, y)],
+                             fn = ft[IDX(q, x, yp)], fw = ft[IDX(q, xm, y)],
+                             fs = ft[IDX(q, x, ym)], fne = ft[IDX(q, xp, yp)],
+                             fnw = ft[IDX(q, xm, yp)], fsw = ft[IDX(q, xm, ym)],
+                             fse = ft[IDX(q, xp, ym)];
+                const double cfw = (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) -
+                                   (p8 * cyq) * (((fnw + fn) - fsw) - fs);
+                const double cfn = (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) -
+                                   (p8 * cxq) * (((fne + fe) - fnw) - fw);
+                const double cfe = (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) -
+                                   (p8 * cyq) * (((fne + fn) - fse) - fs);
+                const double cfs = (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) -
+                                   (p8 * cxq) * (((fse + fe) - fsw) - fw);
+                const size_t i = IDX(q, x, y);
+                fp[i] = (fp[i] - cxq * (cfe - cfw)) - cyq * (cfn - cfs);
+            }
+        }
+    }
+}
 #undef IDX
 
 /* ------------------------------------------------------------------ collisions */

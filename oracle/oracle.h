@@ -38,6 +38,10 @@ void orc_stream(int nx, int ny, int ld, const double *fsrc, double *fdst);
 void orc_stream_fvm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
 void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
 void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
+/* DUGKS, src/periodic_dugks.F90 (default build) */
+void orc_dugks_collide(int nx, int ny, int ld, double *fold, double *fnew, double omega_full,
+                       double omega_half);
+void orc_dugks_stream(int nx, int ny, int ld, const double *ft, double *fp, double dt);
 void orc_collide_bgk(int nx, int ny, int ld, double *f, double omega);
 void orc_collide_trt(int nx, int ny, int ld, double *f, double lambda_e, double lambda_d);
 void orc_collide_rr(int nx, int ny, int ld, double *f, double omega);

@@ -74,6 +74,8 @@ class Oracle:
         for name in ("orc_stream_fvm_bardow", "orc_stream_fdm_bardow", "orc_stream_fdm_sofonea"):
             getattr(L, name).argtypes = [_i, _i, _i, _dp, _dp, _d]
         L.orc_collide_bgk.argtypes = [_i, _i, _i, _dp, _d]
+        L.orc_dugks_collide.argtypes = [_i, _i, _i, _dp, _dp, _d, _d]
+        L.orc_dugks_stream.argtypes = [_i, _i, _i, _dp, _dp, _d]
         L.orc_collide_trt.argtypes = [_i, _i, _i, _dp, _d, _d]
         L.orc_collide_rr.argtypes = [_i, _i, _i, _dp, _d]
         L.orc_collide.argtypes = [_i, _i, _i, _i, _dp, _d, _d]
@@ -127,6 +129,24 @@ class Oracle:
 
     def collide_trt(self, nx, ny, f, lambda_e, lambda_d):
         self.lib.orc_collide_trt(nx, ny, f.shape[0], _ptr(f), lambda_e, lambda_d)
+
+    def dugks_collide(self, nx, ny, fold, fnew, omega_full, omega_half):
+        """dugks_collide: fnew = BGK(fold, omega_full); fold = BGK(fold, omega_half)."""
+        self.lib.orc_dugks_collide(nx, ny, fold.shape[0], _ptr(fold), _ptr(fnew), omega_full,
+                                   omega_half)
+
+    def dugks_stream(self, nx, ny, ft, fp, dt):
+        """dugks_stream: fp -= flux balance of the face values of ft."""
+        self.lib.orc_dugks_stream(nx, ny, ft.shape[0], _ptr(ft), _ptr(fp), dt)
+
+    def dugks_run(self, nx, ny, fa, fb, nsteps, dt, omega_full, omega_half):
+        """perform_dugks_step x nsteps on (fa = iold, fb = inew), with the reference's
+        buffer swap; returns (iold, inew) after the last step."""
+        for _ in range(nsteps):
+            self.dugks_collide(nx, ny, fa, fb, omega_full, omega_half)
+            self.dugks_stream(nx, ny, fa, fb, dt)
+            fa, fb = fb, fa
+        return fa, fb
 
     def collide_shifted(self, model, nx, ny, g, omega, lambda_odd=0.0):
         self.lib.orc_collide_shifted(MODELS.get(model, model), nx, ny, g.shape[0], _ptr(g),
@@ -202,6 +222,8 @@ class RefLib:
         L.ref_set_streaming.argtypes = [ctypes.c_void_p, _i]
         L.ref_streaming.argtypes = [ctypes.c_void_p]
         L.ref_perform_step.argtypes = [ctypes.c_void_p, _i]
+        L.ref_set_dugks.argtypes = [ctypes.c_void_p]
+        L.ref_perform_dugks_step.argtypes = [ctypes.c_void_p, _i]
 
     def equilibrium(self, rho, ux, uy):
         out = np.zeros(9)
@@ -295,6 +317,13 @@ class RefGrid:
 
     def collide(self):
         self.ref.lib.ref_collide(self.h)
+
+    def set_dugks(self):
+        """grid%collision => dugks_collide, grid%streaming => dugks_stream."""
+        self.ref.lib.ref_set_dugks(self.h)
+
+    def dugks_step(self, n=1):
+        self.ref.lib.ref_perform_dugks_step(self.h, n)
 
     def update_macros(self):
         self.ref.lib.ref_update_macros(self.h)

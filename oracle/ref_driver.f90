@@ -13,6 +13,8 @@
 !   collide_rr                           src/collision_regularized.F90:21-202
 !   taylor_green_t%eval                  src/benchmarks/taylor_green.f90:31-84
 !   shear_layer_t%eval                   src/benchmarks/shear_layer.f90:123-150
+!   perform_dugks_step, dugks_collide,
+!   dugks_stream                         src/periodic_dugks.F90:25-299
 module ref_driver
 
    use iso_c_binding
@@ -24,6 +26,7 @@ module ref_driver
    use taylor_green, only: taylor_green_t
    use shear_layer_case, only: shear_layer_t
    use barotropic_vortex_case, only: vortex_case_t
+   use periodic_dugks, only: perform_dugks_step, dugks_collide, dugks_stream
 
    implicit none
 
@@ -200,6 +203,25 @@ contains
       p => unbox(h)
       do s = 1, nsteps
          call perform_step(p%grid)
+      end do
+   end subroutine
+
+   subroutine ref_set_dugks(h) bind(c, name="ref_set_dugks")
+      type(c_ptr), value :: h
+      type(grid_box), pointer :: p
+      p => unbox(h)
+      p%grid%collision => dugks_collide
+      p%grid%streaming => dugks_stream
+   end subroutine
+
+   subroutine ref_perform_dugks_step(h, nsteps) bind(c, name="ref_perform_dugks_step")
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+      type(grid_box), pointer :: p
+      integer :: s
+      p => unbox(h)
+      do s = 1, nsteps
+         call perform_dugks_step(p%grid)
       end do
    end subroutine
 

@@ -24,11 +24,13 @@ __all__ = [
     "w",
 ]
 from .lattice import (  # noqa: E402
-    VortexCase, stream_fdm_bardow, stream_fdm_sofonea, stream_fvm_bardow,
+    VortexCase, dugks_collide, dugks_stream, perform_dugks_step, stream_fdm_bardow,
+    stream_fdm_sofonea, stream_fvm_bardow, tau_over_dt,
 )
 from .batch import Batch, Case, sweep_cases  # noqa: E402
 
-__all__ += ["VortexCase", "stream_fdm_bardow", "stream_fdm_sofonea", "stream_fvm_bardow"]
+__all__ += ["VortexCase", "dugks_collide", "dugks_stream", "perform_dugks_step",
+            "stream_fdm_bardow", "stream_fdm_sofonea", "stream_fvm_bardow", "tau_over_dt"]
 
 __all__ += ["Batch", "Case", "sweep_cases"]
 from .adjoint import Adjoint, band_limited_field, velocity_misfit  # noqa: E402

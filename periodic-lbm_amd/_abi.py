@@ -104,6 +104,9 @@ SIGNATURES = {
     "plbm_adjoint_backward_dev": (_i, [_vp, _vp, _vp, _vp, _dp]),
     "plbm_field_band_limited": (_i, [_i, _i, _i, _d, _u64, _dp, _dp]),
     "plbm_stream_op": (_i, [_vp, _i, _i, _i, _d]),
+    "plbm_dugks_collide": (_i, [_vp, _i, _i, _d, _d]),
+    "plbm_dugks_stream": (_i, [_vp, _i, _i, _d]),
+    "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),
     "plbm_field_barotropic_vortex": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _d, _dp, _dp, _dp]),
 }
 STREAM_OPS = {"exact": 0, "fvm_bardow": 1, "fdm_bardow": 2, "fdm_sofonea": 3}

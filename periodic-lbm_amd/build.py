@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip"]
+HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip", "dugks.hip"]
 CXX_SOURCES = ["fields.cpp", "sim_plugin.cpp"]
 # -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
 # reference is built without FMA); kernels that want FMA enable it locally by pragma.
@@ -45,7 +45,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-Wa
 FORTRAN_SOURCES = [
     "plbm_capi.f90", "precision.F90", "fvm_bardow.F90", "periodic_lbm.f90",
     "collision_bgk.f90", "collision_trt.f90", "collision_regularized.f90",
-    "taylor_green.f90", "shear_layer.f90", "barotropic_vortex_case.f90",
+    "taylor_green.f90", "shear_layer.f90", "barotropic_vortex_case.f90", "periodic_dugks.f90",
 ]
 
 
@@ -111,7 +111,7 @@ def link_fortran_program(src: Path, out: Path, moddir: Path | None = None) -> Pa
 def build_fortran_tests(verbose: bool = True) -> None:
     """The Fortran test programs under tests/fortran, linked against the shim."""
     out = PKG / "_build" / "tests"
-    for name in ("shim_check", "outputs_check"):
+    for name in ("shim_check", "outputs_check", "dugks_check"):
         src = ROOT / "tests" / "fortran" / f"{name}.f90"
         if src.exists():
             link_fortran_program(src, out / name, moddir=out / f"mod_{name}")

@@ -40,6 +40,8 @@ module fvm_bardow
    public :: plbm_sync_host, plbm_sync_device, plbm_fused_model
    public :: plbm_lbm_stream, plbm_collide_bgk, plbm_collide_trt, plbm_collide_rr
    public :: plbm_reductions
+   public :: plbm_grid_dugks_collide, plbm_grid_dugks_stream, plbm_grid_perform_dugks_step
+   public :: plbm_tau_over_dt
    public :: PLBM_F64, PLBM_F64S, PLBM_F32S, PLBM_F16S
 
    integer, parameter :: WHERE_HOST = 0, WHERE_DEVICE = 1
@@ -356,14 +358,16 @@ contains
       yes = associated(grid%streaming, plbm_lbm_stream) .or. &
             associated(grid%streaming, stream_fvm_bardow) .or. &
             associated(grid%streaming, stream_fdm_bardow) .or. &
-            associated(grid%streaming, stream_fdm_sofonea)
+            associated(grid%streaming, stream_fdm_sofonea) .or. &
+            associated(grid%streaming, plbm_grid_dugks_stream)
    end function
 
    logical function is_engine_op_c(grid) result(yes)
       type(lattice_grid), intent(in) :: grid
       yes = associated(grid%collision, plbm_collide_bgk) .or. &
             associated(grid%collision, plbm_collide_trt) .or. &
-            associated(grid%collision, plbm_collide_rr)
+            associated(grid%collision, plbm_collide_rr) .or. &
+            associated(grid%collision, plbm_grid_dugks_collide)
    end function
 
    !> Call grid%streaming, moving the populations to the side the operator works on.
@@ -505,6 +509,59 @@ contains
    subroutine stream_fdm_sofonea(grid)
       class(lattice_grid), intent(inout) :: grid
       call stream_op_on_device(grid, PLBM_STREAM_FDM_SOFONEA, "stream_fdm_sofonea")
+   end subroutine
+
+   ! ------------------------------------------------------------------ DUGKS
+
+   !> tau_over_dt (src/periodic_dugks.F90:40-44).
+   pure function plbm_tau_over_dt(grid) result(tau_d)
+      type(lattice_grid), intent(in) :: grid
+      real(wp) :: tau_d
+      tau_d = grid%tau/grid%dt
+   end function
+
+   !> dugks_collide (src/periodic_dugks.F90:46-77) on the device:
+   !> f(inew) = BGK(f(iold), grid%omega), f(iold) = BGK(f(iold), 1/(tau/dt + 1/2)).
+   subroutine plbm_grid_dugks_collide(grid)
+      class(lattice_grid), intent(inout) :: grid
+      call require_device(grid)
+      call plbm_check(plbm_dugks_collide(grid%dev, int(grid%iold-1,c_int), &
+         int(grid%inew-1,c_int), grid%omega, 1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), &
+         "dugks_collide")
+      call device_written(grid)
+   end subroutine
+
+   !> dugks_stream (src/periodic_dugks.F90:168-299) on the device: f(inew) minus the
+   !> face-flux balance of f(iold).
+   subroutine plbm_grid_dugks_stream(grid)
+      class(lattice_grid), intent(inout) :: grid
+      call require_device(grid)
+      call plbm_check(plbm_dugks_stream(grid%dev, int(grid%iold-1,c_int), &
+         int(grid%inew-1,c_int), grid%dt), "dugks_stream")
+      call device_written(grid)
+   end subroutine
+
+   !> perform_dugks_step (src/periodic_dugks.F90:25-38): collision, streaming, swap.  With
+   !> dugks_collide + dugks_stream in the slots both run as the engine's fused step.
+   subroutine plbm_grid_perform_dugks_step(grid)
+      type(lattice_grid), intent(inout) :: grid
+      integer :: itmp
+
+      if (associated(grid%collision, plbm_grid_dugks_collide) .and. &
+          associated(grid%streaming, plbm_grid_dugks_stream)) then
+         call require_device(grid)
+         call plbm_check(plbm_dugks_step(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), 1_c_int, grid%dt, grid%omega, &
+            1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), "perform_dugks_step")
+         call device_written(grid)
+      else
+         call dispatch_collision(grid)
+         call dispatch_streaming(grid)
+      end if
+
+      itmp = grid%iold
+      grid%iold = grid%inew
+      grid%inew = itmp
    end subroutine
 
    ! ------------------------------------------------------------------ output

@@ -15,6 +15,7 @@ module plbm_capi
    public :: plbm_create, plbm_destroy, plbm_set_params, plbm_upload_f, plbm_download_f
    public :: plbm_equilibrium_init, plbm_step, plbm_run, plbm_stream, plbm_collide, plbm_copy
    public :: plbm_stream_op
+   public :: plbm_dugks_collide, plbm_dugks_stream, plbm_dugks_step
    integer(c_int), parameter, public :: PLBM_STREAM_EXACT = 0, PLBM_STREAM_FVM_BARDOW = 1, &
                                         PLBM_STREAM_FDM_BARDOW = 2, PLBM_STREAM_FDM_SOFONEA = 3
    public :: plbm_macros, plbm_reduce, plbm_sync, plbm_last_error, plbm_check
@@ -84,6 +85,29 @@ module plbm_capi
          type(c_ptr), value :: h
          integer(c_int), value :: op, src, dst
          real(c_double), value :: dt
+      end function
+
+      integer(c_int) function plbm_dugks_collide(h, src, dst, omega_full, omega_half) &
+            bind(c, name="plbm_dugks_collide")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: src, dst
+         real(c_double), value :: omega_full, omega_half
+      end function
+
+      integer(c_int) function plbm_dugks_stream(h, src, dst, dt) bind(c, name="plbm_dugks_stream")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: src, dst
+         real(c_double), value :: dt
+      end function
+
+      integer(c_int) function plbm_dugks_step(h, iold, inew, nsteps, dt, omega_full, &
+            omega_half) bind(c, name="plbm_dugks_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: iold, inew, nsteps
+         real(c_double), value :: dt, omega_full, omega_half
       end function
 
       integer(c_int) function plbm_collide(h, model, buf) bind(c, name="plbm_collide")

@@ -10,6 +10,8 @@ Names, argument meaning and buffer bookkeeping follow ivan-pi/periodic-lbm:
   collide_rr                                                          src/collision_regularized.F90
   TaylorGreen (taylor_green_t) / pi                                   src/benchmarks/taylor_green.f90
   ShearLayer (shear_layer_t)                                          src/benchmarks/shear_layer.f90
+  stream_fvm_bardow / stream_fdm_bardow / stream_fdm_sofonea          src/fvm_bardow.F90
+  perform_dugks_step / dugks_collide / dugks_stream / tau_over_dt     src/periodic_dugks.F90
 
 Host arrays use the reference's layouts: grid.f is f(ld, nx, 0:8, nf) and grid.mf is
 mf(ny, nx, 3), both Fortran-ordered numpy arrays; grid.rho / ux / uy are views of mf.
@@ -141,6 +143,20 @@ class Lattice:
 
     def collide(self, model: str | int, buf: int):
         check(lib().plbm_collide(self.h, _abi.MODEL_IDS.get(model, model), buf))
+
+    def dugks_collide(self, src: int, dst: int, omega_full: float, omega_half: float):
+        """dugks_collide: dst = BGK(src, omega_full), src = BGK(src, omega_half)."""
+        check(lib().plbm_dugks_collide(self.h, src, dst, omega_full, omega_half))
+
+    def dugks_stream(self, src: int, dst: int, dt: float):
+        """dugks_stream: dst -= dt c_q . face flux balance of src."""
+        check(lib().plbm_dugks_stream(self.h, src, dst, dt))
+
+    def dugks_step(self, iold: int, inew: int, nsteps: int, dt: float, omega_full: float,
+                   omega_half: float):
+        """nsteps x perform_dugks_step on slots (iold, inew); slot contents afterwards
+        are the reference's buffers of the same index (ids swap once per step)."""
+        check(lib().plbm_dugks_step(self.h, iold, inew, nsteps, dt, omega_full, omega_half))
 
     def copy(self, src: int, dst: int):
         check(lib().plbm_copy(self.h, src, dst))
@@ -346,8 +362,49 @@ def collide_rr(grid: LatticeGrid) -> None:
     _collide(grid, _abi.RR)
 
 
-_ENGINE_COLLISIONS = {collide_bgk: _abi.BGK, collide_trt: _abi.TRT, collide_rr: _abi.RR}
-_ENGINE_STREAMING = (lbm_stream, stream_fvm_bardow, stream_fdm_bardow, stream_fdm_sofonea)
+def tau_over_dt(grid: LatticeGrid) -> float:
+    """tau_over_dt (src/periodic_dugks.F90:40-44)."""
+    return grid.tau / grid.dt
+
+
+def _dugks_omega_half(grid: LatticeGrid) -> float:
+    return 1.0 / (tau_over_dt(grid) + 0.5)  # src/periodic_dugks.F90:56
+
+
+def dugks_collide(grid: LatticeGrid) -> None:
+    """dugks_collide (src/periodic_dugks.F90:46-77) on the device: f(inew) = BGK(f(iold),
+    grid.omega), f(iold) = BGK(f(iold), 1/(tau/dt + 1/2)) in place."""
+    _require_device(grid)
+    grid.lattice.dugks_collide(grid.iold - 1, grid.inew - 1, grid.omega, _dugks_omega_half(grid))
+    _device_written(grid)
+
+
+def dugks_stream(grid: LatticeGrid) -> None:
+    """dugks_stream (src/periodic_dugks.F90:168-299) on the device: f(inew) minus the
+    face-flux balance of f(iold), time step grid.dt."""
+    _require_device(grid)
+    grid.lattice.dugks_stream(grid.iold - 1, grid.inew - 1, grid.dt)
+    _device_written(grid)
+
+
+def perform_dugks_step(grid: LatticeGrid) -> None:
+    """perform_dugks_step (src/periodic_dugks.F90:25-38): grid.collision, grid.streaming,
+    swap.  With dugks_collide + dugks_stream in the slots the step runs fused."""
+    if grid.collision is dugks_collide and grid.streaming is dugks_stream:
+        _require_device(grid)
+        grid.lattice.dugks_step(grid.iold - 1, grid.inew - 1, 1, grid.dt, grid.omega,
+                                _dugks_omega_half(grid))
+        _device_written(grid)
+    else:
+        _dispatch(grid, grid.collision, _ENGINE_COLLISIONS)
+        _dispatch(grid, grid.streaming, _ENGINE_STREAMING)
+    grid.iold, grid.inew = grid.inew, grid.iold
+
+
+_ENGINE_COLLISIONS = {collide_bgk: _abi.BGK, collide_trt: _abi.TRT, collide_rr: _abi.RR,
+                      dugks_collide: None}
+_ENGINE_STREAMING = (lbm_stream, stream_fvm_bardow, stream_fdm_bardow, stream_fdm_sofonea,
+                     dugks_stream)
 
 
 def _dispatch(grid: LatticeGrid, op, engine_ops):

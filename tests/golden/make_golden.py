@@ -11,6 +11,11 @@ data files in this directory:
                   fingerprint), the relative L2 errors vs the analytic solution, mass
   ref_params.json set_properties / lambda_d / magic_number / equilibrium values
   ref_app.json    the unchanged app's printed results (main_taylor_green 1.0)
+  ref_streaming_ops.npz / ref_vortex_app.json   FV/FD streaming operators, main_vortex
+  ref_outputs.json  sha256 of the VTK / npy / gnuplot files of tests/fortran/outputs_check
+  ref_dugks.npz   DUGKS (src/periodic_dugks.F90): both buffers after 1 and 7 steps and
+                  the update_macros fields, square and non-square, dt = 1 and 0.5
+  ref_dugks_app.json  sha256 of the raw fields tests/fortran/dugks_check.f90 writes
 
 Usage (in a container that has the reference tree and flang):
   make -C oracle ref && python tests/golden/make_golden.py [--app]
@@ -174,6 +179,51 @@ def vortex_app():
         {"args": ["1.0"], "stdout": r.stdout, "files": files}, indent=1))
 
 
+def dugks(R):
+    """perform_dugks_step with dugks_collide + dugks_stream in the slots."""
+    out = {}
+    cases = [(16, 16, 0.02, 1.0), (12, 20, 0.01, 0.5), (24, 8, 0.005, 2.0)]
+    for nx, ny, nu, dt in cases:
+        key = f"{nx}x{ny}_dt{dt}"
+        rng = np.random.default_rng(nx * 31 + ny)
+        rho = np.asfortranarray(1.0 + 0.01 * rng.standard_normal((ny, nx)))
+        ux = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        uy = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+        g = R.grid(nx, ny, "bgk")
+        g.set_properties(nu, dt, 0.25)
+        g.set_dugks()
+        g.set_macros(rho, ux, uy)
+        g.set_pdf_to_equilibrium()
+        par = g.params()
+        out[f"{key}_params"] = np.array([nu, dt, par["tau"], par["omega"]])
+        done = 0
+        for n in (1, 7):
+            g.dugks_step(n - done)
+            done = n
+            iold, inew, _, _ = g.indices()
+            out[f"{key}_{n}_iold"] = g.get_f(iold)[:ny]
+            out[f"{key}_{n}_inew"] = g.get_f(inew)[:ny]
+            out[f"{key}_{n}_ids"] = np.array([iold, inew])
+            g.update_macros()
+            r, u, v = g.macros()
+            out[f"{key}_{n}_macros"] = np.stack([r, u, v], axis=2)
+        g.close()
+    np.savez_compressed(HERE / "ref_dugks.npz", **out)
+
+
+def dugks_app():
+    """sha256 of the raw fields that tests/fortran/dugks_check.f90 writes through the
+    reference modules (oracle/_ref/dugks_check_ref)."""
+    exe = po.HERE / "_ref" / "dugks_check_ref"
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, check=True)
+        files = {p.name: hashlib.sha256(p.read_bytes()).hexdigest()
+                 for p in sorted(Path(d, "out").iterdir())}
+    (HERE / "ref_dugks_app.json").write_text(json.dumps({"stdout": r.stdout, "files": files},
+                                                        indent=1))
+
+
 def outputs():
     """sha256 of the files tests/fortran/outputs_check.f90 writes through the reference's
     output_vtk / output_npy / output_gnuplot (oracle/_ref/outputs_check_ref)."""
@@ -187,6 +237,10 @@ def outputs():
 
 if __name__ == "__main__":
     R = po.RefLib()
+    if "--dugks-only" in sys.argv:
+        dugks(R)
+        dugks_app()
+        sys.exit(0)
     if "--streaming-only" in sys.argv:
         streaming_ops(R)
         vortex_app()
@@ -196,6 +250,8 @@ if __name__ == "__main__":
         small(R)
         config1(R)
         streaming_ops(R)
+        dugks(R)
+        dugks_app()
         if "--app" in sys.argv:
             app()
             vortex_app()

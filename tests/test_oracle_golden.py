@@ -103,3 +103,34 @@ def test_streaming_operators_bitwise(oracle, op, dt):
         oracle.collide("bgk", n, n, b, omega)
         a, b = b, a
     np.testing.assert_array_equal(a[:n], G[f"{op}_dt{dt}"])
+
+
+DUGKS_CASES = [(16, 16, 1.0), (12, 20, 0.5), (24, 8, 2.0)]
+
+
+@pytest.mark.parametrize("nx,ny,dt", DUGKS_CASES)
+def test_dugks_bitwise(oracle, nx, ny, dt):
+    """perform_dugks_step (src/periodic_dugks.F90:25-38): both buffers after 1 and 7 steps
+    and update_macros of f(inew), against the reference's own run (ref_dugks.npz)."""
+    G = np.load(GOLDEN / "ref_dugks.npz")
+    key = f"{nx}x{ny}_dt{dt}"
+    rho, ux, uy = (np.asfortranarray(G[f"{key}_{c}"]) for c in ("rho", "ux", "uy"))
+    nu, dt_, tau, omega = G[f"{key}_params"]
+    par = oracle.set_properties(nu, dt, 0.25)
+    assert (par["tau"], par["omega"]) == (tau, omega)
+    om_half = 1.0 / (tau / dt + 0.5)
+    bufs = [po.new_f(nx, ny), po.new_f(nx, ny)]
+    inew, iold = (int(i) for i in G[f"{key}_1_ids"])  # alloc_grid's ids, one swap back
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, bufs[iold - 1])
+    done = 0
+    for n in (1, 7):
+        for _ in range(n - done):
+            oracle.dugks_collide(nx, ny, bufs[iold - 1], bufs[inew - 1], omega, om_half)
+            oracle.dugks_stream(nx, ny, bufs[iold - 1], bufs[inew - 1], dt)
+            iold, inew = inew, iold
+        done = n
+        assert [iold, inew] == list(G[f"{key}_{n}_ids"])
+        np.testing.assert_array_equal(bufs[iold - 1][:ny], G[f"{key}_{n}_iold"])
+        np.testing.assert_array_equal(bufs[inew - 1][:ny], G[f"{key}_{n}_inew"])
+        r, u, v = oracle.update_macros(nx, ny, bufs[inew - 1])
+        np.testing.assert_array_equal(np.stack([r, u, v], axis=2), G[f"{key}_{n}_macros"])

@@ -313,3 +313,99 @@ def test_fd_streaming_conserves_and_keeps_constants(oracle, op):
     np.testing.assert_allclose(out[:ny], 0.37, rtol=1e-15)
     oracle.stream_op(op, nx, ny, f, out, 0.0)  # dt = 0: identity
     np.testing.assert_array_equal(out[:ny], f[:ny])
+
+
+# ---------------------------------------------------------------- DUGKS
+
+
+def _random_f(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    rho = 1 + 0.02 * rng.standard_normal((ny, nx))
+    ux, uy = 0.05 * rng.standard_normal((ny, nx)), 0.05 * rng.standard_normal((ny, nx))
+    f = po.new_f(nx, ny)
+    feq = np.stack([np.stack([po_feq(r, u, v) for r, u, v in zip(rho[:, x], ux[:, x], uy[:, x])])
+                    for x in range(nx)], axis=1)
+    f[:ny] = feq * (1 + 0.01 * rng.standard_normal(feq.shape))
+    return f
+
+
+def po_feq(rho, ux, uy):
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    c = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
+    cu = c[:, 0] * ux + c[:, 1] * uy
+    return w * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux * ux + uy * uy))
+
+
+def test_dugks_collision_is_bgk(oracle):
+    """kernel_bgk (src/periodic_dugks.F90:80-165) is BGK in the split-weight algebra: it
+    agrees with collide_bgk to roundoff, omega = 1 gives the equilibrium, and both
+    outputs keep each cell's mass and momentum."""
+    nx, ny = 7, 5
+    f = _random_f(nx, ny, 11)
+    fold, fnew = f.copy(order="F"), po.new_f(nx, ny)
+    oracle.dugks_collide(nx, ny, fold, fnew, 1.7, 0.9)
+    for om, got in ((1.7, fnew), (0.9, fold)):
+        want = f.copy(order="F")
+        oracle.collide("bgk", nx, ny, want, om)
+        np.testing.assert_allclose(got[:ny], want[:ny], rtol=0, atol=1e-15)
+        cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+        cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+        np.testing.assert_allclose(got[:ny].sum(axis=2), f[:ny].sum(axis=2), rtol=1e-15)
+        np.testing.assert_allclose(got[:ny] @ cx, f[:ny] @ cx, atol=1e-16)
+        np.testing.assert_allclose(got[:ny] @ cy, f[:ny] @ cy, atol=1e-16)
+    one, dummy = f.copy(order="F"), po.new_f(nx, ny)
+    oracle.dugks_collide(nx, ny, one, dummy, 1.0, 1.0)
+    r, u, v = oracle.update_macros(nx, ny, f)
+    for x in range(nx):
+        for y in range(ny):
+            np.testing.assert_allclose(one[y, x], po_feq(r[y, x], u[y, x], v[y, x]),
+                                       rtol=0, atol=2e-16)
+
+
+def test_dugks_stream_is_fvm_flux_form(oracle):
+    """kernel_stream with fp = ft is stream_fvm_bardow's update (same face values, same
+    balance), so it matches that reference-pinned operator bit for bit; q = 0 is untouched,
+    mass per direction is conserved, constants and dt = 0 leave fp unchanged."""
+    nx, ny = 9, 7
+    rng = np.random.default_rng(5)
+    ft = po.new_f(nx, ny)
+    ft[:ny] = rng.random((ny, nx, 9))
+    fp = ft.copy(order="F")
+    oracle.dugks_stream(nx, ny, ft, fp, 0.8)
+    fv = po.new_f(nx, ny)
+    oracle.stream_op("fvm_bardow", nx, ny, ft, fv, 0.8)
+    np.testing.assert_array_equal(fp[:ny], fv[:ny])
+    other = po.new_f(nx, ny)
+    other[:ny] = rng.random((ny, nx, 9))
+    before = other.copy(order="F")
+    oracle.dugks_stream(nx, ny, ft, other, 0.8)
+    np.testing.assert_array_equal(other[:ny, :, 0], before[:ny, :, 0])
+    np.testing.assert_allclose(other[:ny].sum(axis=(0, 1)), before[:ny].sum(axis=(0, 1)),
+                               rtol=1e-13)
+    const = po.new_f(nx, ny)
+    const[:ny] = 0.37
+    other = before.copy(order="F")
+    oracle.dugks_stream(nx, ny, const, other, 0.8)
+    np.testing.assert_allclose(other[:ny], before[:ny], rtol=1e-14)
+    other = before.copy(order="F")
+    oracle.dugks_stream(nx, ny, ft, other, 0.0)
+    np.testing.assert_array_equal(other[:ny], before[:ny])
+
+
+@pytest.mark.parametrize("q,c", [(2, (0, 1)), (5, (1, 1)), (6, (-1, 1))])
+def test_dugks_stream_is_second_order_advection(oracle, q, c):
+    """With fp = ft, one stream step advects a Fourier mode by dt c_q with a local error
+    of third order in k (halving k divides it by ~8): a wrong sign or neighbour in a face
+    value or cross term drops the order."""
+    n, dt = 64, 0.7
+    x, y = np.arange(n)[None, :], np.arange(n)[:, None]
+    errs = []
+    for m in (1, 2, 4):
+        kx, ky = 2 * np.pi * m / n, 2 * np.pi * 2 * m / n
+        ft = po.new_f(n, n)
+        ft[:n, :, q] = np.cos(kx * x + ky * y)
+        fp = ft.copy(order="F")
+        oracle.dugks_stream(n, n, ft, fp, dt)
+        exact = np.cos(kx * (x - c[0] * dt) + ky * (y - c[1] * dt))
+        errs.append(np.abs(fp[:n, :, q] - exact).max())
+    assert 7.0 < errs[1] / errs[0] < 8.5 and 7.0 < errs[2] / errs[1] < 8.5, errs
