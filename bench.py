@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000,
                     help="steps of the batched uneven-size sweep (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--next-steps", type=int, default=50,
+                    help="timed steps of the DUGKS and FV/FD streaming legs (0 skips them)")
     return ap.parse_args()
 
 
@@ -131,10 +133,64 @@ def gpu_leg(args):
 
     copy_gbs = P.copy_bandwidth(4 << 30, 20)
     extras = {
+        "schemes": schemes_leg(P, torch, args, stream, (d_rho, d_ux, d_uy))
+        if args.next_steps > 0 else None,
         "batched_sweep": batch_leg(P, torch, args) if args.batch_steps > 0 else None,
         "adjoint": None if args.no_adjoint else adjoint_leg(P, torch, args),
     }
     return results, copy_gbs, dev_name, extras
+
+
+def schemes_leg(P, torch, args, stream, fields):
+    """The other schemes on the same 8192^2 FP64 start: DUGKS (src/periodic_dugks.F90) as
+    the fused two-kernel step and as the reference's collide + stream sequence, and the
+    finite-volume / finite-difference streaming operators (src/fvm_bardow.F90:393-893)
+    with collide_bgk as perform_step runs them (one fused pass).  Algorithmic bytes per
+    cell and step: fused DUGKS 360 (collide: read 72 + write 72; flux: read ft 72 + f 72,
+    write 72), reference-form DUGKS 432, stream operator + BGK 144."""
+    n = args.n
+    cells = n * n
+    tau = 3.0 * NU
+    om_full, om_half = 1.0 / (tau + 0.5), 1.0 / (tau + 0.5)
+    lat = P.Lattice(n, n, 2, "f64")
+    lat.set_stream(stream)
+    lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, steps, warm):
+        fn(warm)
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        fn(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def rec(ms, bpc):
+        return {"ms_per_step": round(ms, 5), "mlups": round(cells / ms / 1e3, 1),
+                "bytes_per_cell": bpc, "gbs": round(bpc * cells / ms / 1e6, 1),
+                "frac_of_peak": round(bpc * cells / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+    steps = args.next_steps
+    out = {"grid": [n, n], "storage": "f64", "steps": steps}
+    ms = timed(lambda k: lat.dugks_step(0, 1, k, 1.0, om_full, om_half), steps, 4)
+    out["dugks_fused"] = rec(ms, 360)
+
+    def dugks_ref(k):
+        for s in range(k):
+            lat.dugks_collide(s % 2, 1 - s % 2, om_full, om_half)
+            lat.dugks_stream(s % 2, 1 - s % 2, 1.0)
+    out["dugks_reference_form"] = rec(timed(dugks_ref, 10, 2), 432)
+    lat.set_params(om_full, MAGIC)
+    for op in ("fvm_bardow", "fdm_bardow", "fdm_sofonea"):
+        def run(k, op=op):
+            for s in range(k):
+                lat.step_op(op, "bgk", s % 2, 1 - s % 2, 1.0)
+        out[f"stream_{op}+bgk"] = rec(timed(run, steps, 2), 144)
+    lat.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def batch_leg(P, torch, args):
@@ -323,6 +379,7 @@ def main():
         "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
         "batched_sweep": extras["batched_sweep"],
         "adjoint": extras["adjoint"],
+        "schemes": extras["schemes"],
     }
     print(json.dumps(line), flush=True)
 

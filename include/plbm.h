@@ -137,6 +137,12 @@ int plbm_copy(plbm_lattice *h, int src, int dst);
 #define PLBM_STREAM_FDM_BARDOW 2
 #define PLBM_STREAM_FDM_SOFONEA 3
 int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
+/* perform_step (src/fvm_bardow.F90:307-320) with grid%streaming = streaming operator `op`
+ * and grid%collision = collide_<model>, fused into one pass: src -> stream op -> collide ->
+ * dst.  In FP64 storage bit-identical to plbm_stream_op followed by plbm_collide (the
+ * unfused path stores the intermediate exactly); in FP32/FP16 storage it skips that
+ * intermediate rounding.  op = PLBM_STREAM_EXACT is plbm_step.  Asynchronous. */
+int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt);
 
 /* DUGKS, src/periodic_dugks.F90 (default build, without -DDUGKS), FP64 storage only
  * (PLBM_ERR_STATE otherwise).  omega_full is grid%omega, omega_half the reference's

@@ -104,6 +104,7 @@ SIGNATURES = {
     "plbm_adjoint_backward_dev": (_i, [_vp, _vp, _vp, _vp, _dp]),
     "plbm_field_band_limited": (_i, [_i, _i, _i, _d, _u64, _dp, _dp]),
     "plbm_stream_op": (_i, [_vp, _i, _i, _i, _d]),
+    "plbm_step_op": (_i, [_vp, _i, _i, _i, _i, _d]),
     "plbm_dugks_collide": (_i, [_vp, _i, _i, _d, _d]),
     "plbm_dugks_stream": (_i, [_vp, _i, _i, _d]),
     "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),

@@ -96,6 +96,17 @@ struct StepArgs {
     float scale, inv_scale; // FP16 storage scale (1 otherwise)
 };
 
+// Strip length (columns per thread) of the stencil kernels that walk along x with a
+// register window: 32 (measured best at 8192^2), shorter where that would leave too few
+// 256-thread workgroups for the 256 CUs (2 resident per CU at those kernels' register
+// count: 512 fill the chip once).
+inline int strip_columns(int nx, int ny) {
+    const long nby = (ny + 255) / 256;
+    if (nby * ((nx + 31) / 32) >= 1024) return 32;
+    if (nby * ((nx + 7) / 8) >= 512) return 8;
+    return 1;
+}
+
 // Compile-time dispatch of a runtime storage / model id to a templated callable.
 template <class F>
 inline int dispatch_storage(int st, F &&f) {

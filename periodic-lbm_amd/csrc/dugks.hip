@@ -10,9 +10,10 @@
 // Two kernels carry each form:
 //   k_dugks_collide   per cell: \bar f^+ into `half` (in place on the source for the
 //                     reference's dugks_collide) and optionally BGK(fold, omega) into `full`
-//   k_dugks_flux      per cell: 9-point stencil of \bar f^+ per direction, flux balance
-//                     subtracted from fp; fp is either read (dugks_stream) or recomputed
-//                     as BGK(fold, omega) from the same cell (the fused step)
+//   k_dugks_flux      per row and strip of columns: 9-point stencil of \bar f^+ per
+//                     direction from a register window, flux balance subtracted from fp;
+//                     fp is either read (dugks_stream) or recomputed as BGK(fold, omega)
+//                     from the same cell (the fused step)
 // The fused step (plbm_dugks_step) runs k_dugks_collide without the full collision
 // (\bar f^+ into the other slot) and k_dugks_flux with the collision recomputed from the
 // cell's own old populations, writing f^{n+1} over them in place: 360 bytes per cell per
@@ -25,6 +26,7 @@
 
 #include "common.hpp"
 #include "d2q9.hpp"
+#include "stepper.hpp"
 
 namespace plbm {
 
@@ -55,21 +57,23 @@ __device__ __forceinline__ void dugks_bgk(double f[9], double omega) {
     f[8] = omegabar * f[8] + (om_wd * rho) * (t68 + xmy);
 }
 
-// One thread per cell; blockIdx.y = x, blockIdx.x * blockDim.x + threadIdx.x = y.
-// `half` may equal `src` (each thread reads and writes only its own cell).
+// One thread per cell on a 1-D grid of (ny/256) x nx workgroups, renumbered so that each
+// XCD streams a contiguous range of columns.  `half` may equal `src` (each thread reads
+// and writes only its own cell).
 template <bool FULL>
 __global__ void __launch_bounds__(256)
-    k_dugks_collide(const double *src, double *half, double *full, int ny, int ldd,
+    k_dugks_collide(const double *src, double *half, double *full, int ny, int ldd, int nby,
                     double om_full, double om_half) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int x = b / nby, y = (b % nby) * 256 + threadIdx.x;
     if (y >= ny) return;
-    const size_t L = (size_t)ldd, c0 = (size_t)blockIdx.y * 9 * L + y;
-    double a[9], b[9];
+    const size_t L = (size_t)ldd, c0 = (size_t)x * 9 * L + y;
+    double a[9], h[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) a[q] = b[q] = src[c0 + q * L];
-    dugks_bgk(b, om_half);
+    for (int q = 0; q < 9; ++q) a[q] = h[q] = src[c0 + q * L];
+    dugks_bgk(h, om_half);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) __builtin_nontemporal_store(b[q], half + c0 + q * L);
+    for (int q = 0; q < 9; ++q) __builtin_nontemporal_store(h[q], half + c0 + q * L);
     if constexpr (FULL) {
         dugks_bgk(a, om_full);
 #pragma unroll
@@ -77,49 +81,83 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// kernel_stream (src/periodic_dugks.F90:190-299, default build).  COLLIDE: fp is
-// BGK(fo, om_full) of the thread's own cell (fo may alias fp), all nine directions are
+// Face values and flux balance of one direction (kernel_stream, src/periodic_dugks.F90:
+// 221-233 and 258-270): c = centre, e/n/w/s and corners are the 3x3 neighbourhood of ft.
+__device__ __forceinline__ double dugks_face_update(double fp, double fc, double fe, double fn,
+                                                    double fw, double fs, double fne,
+                                                    double fnw, double fsw, double fse,
+                                                    double cxq, double cyq) {
+#pragma clang fp contract(off)
+    const double p2 = 0.5, p8 = 0.125;
+    const double cfw =
+        (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) - (p8 * cyq) * (((fnw + fn) - fsw) - fs);
+    const double cfn =
+        (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) - (p8 * cxq) * (((fne + fe) - fnw) - fw);
+    const double cfe =
+        (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) - (p8 * cyq) * (((fne + fn) - fse) - fs);
+    const double cfs =
+        (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) - (p8 * cxq) * (((fse + fe) - fsw) - fw);
+    return (fp - cxq * (cfe - cfw)) - cyq * (cfn - cfs);
+}
+
+// kernel_stream (src/periodic_dugks.F90:190-299, default build).  Each thread owns one row
+// y and walks a strip of XS consecutive columns, keeping for every direction the 3x3
+// neighbourhood of ft in registers: a step to the next column loads only the new column's
+// three values per direction (24 loads per cell instead of 72).
+// COLLIDE: fp is BGK(fo, om_full) of the thread's own cell (fo may alias fp: the cell is
+// read before it is written and no other thread reads fo) and all nine directions are
 // written; otherwise fp is read and written for q = 1..8 and q = 0 is left untouched.
-template <bool COLLIDE>
+template <bool COLLIDE, int XS>
 __global__ void __launch_bounds__(256)
     k_dugks_flux(const double *__restrict__ ft, const double *fo, double *fp, int nx, int ny,
                  int ldd, double dt, double om_full) {
-#pragma clang fp contract(off)
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    const int x = blockIdx.y;
+    const int y = blockIdx.x * 256 + threadIdx.x;
+    const int x0 = blockIdx.y * XS;
     if (y >= ny) return;
-    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
     const size_t L = (size_t)ldd, cs = 9 * L;
-    const size_t c0 = (size_t)x * cs, cm = (size_t)xm * cs, cp = (size_t)xp * cs;
-    double g[9];
-    if constexpr (COLLIDE) {
+    auto col = [&](int xx) {
+        xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+        return (size_t)xx * cs;
+    };
+    double w[9][3][3];  // [q][column x-1, x, x+1][row y-1, y, y+1]; q = 0 unused
+    {
+        const size_t cm = col(x0 - 1), c0 = col(x0);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = fo[c0 + q * L + y];
-        dugks_bgk(g, om_full);
-        __builtin_nontemporal_store(g[0], fp + c0 + y);
-    } else {
-#pragma unroll
-        for (int q = 1; q < 9; ++q) g[q] = fp[c0 + q * L + y];
+        for (int q = 1; q < 9; ++q) {
+            const size_t o = (size_t)q * L;
+            w[q][0][0] = ft[cm + o + ym], w[q][0][1] = ft[cm + o + y], w[q][0][2] = ft[cm + o + yp];
+            w[q][1][0] = ft[c0 + o + ym], w[q][1][1] = ft[c0 + o + y], w[q][1][2] = ft[c0 + o + yp];
+        }
     }
-    const double p2 = 0.5, p8 = 0.125;
+    for (int i = 0; i < XS; ++i) {
+        const int x = x0 + i;
+        if (x >= nx) break;
+        const size_t c0 = (size_t)x * cs, cp = col(x + 1);
 #pragma unroll
-    for (int q = 1; q < 9; ++q) {
-        const size_t o = (size_t)q * L;
-        const double fc = ft[c0 + o + y], fe = ft[cp + o + y], fn = ft[c0 + o + yp],
-                     fw = ft[cm + o + y], fs = ft[c0 + o + ym], fne = ft[cp + o + yp],
-                     fnw = ft[cm + o + yp], fsw = ft[cm + o + ym], fse = ft[cp + o + ym];
-        const double cxq = dt * double(cx(q)), cyq = dt * double(cy(q));
-        const double cfw =
-            (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) - (p8 * cyq) * (((fnw + fn) - fsw) - fs);
-        const double cfn =
-            (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) - (p8 * cxq) * (((fne + fe) - fnw) - fw);
-        const double cfe =
-            (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) - (p8 * cyq) * (((fne + fn) - fse) - fs);
-        const double cfs =
-            (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) - (p8 * cxq) * (((fse + fe) - fsw) - fw);
-        __builtin_nontemporal_store((g[q] - cxq * (cfe - cfw)) - cyq * (cfn - cfs),
-                                    fp + c0 + o + y);
+        for (int q = 1; q < 9; ++q) {
+            const size_t o = (size_t)q * L;
+            w[q][2][0] = ft[cp + o + ym], w[q][2][1] = ft[cp + o + y], w[q][2][2] = ft[cp + o + yp];
+        }
+        double g[9];
+        if constexpr (COLLIDE) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) g[q] = fo[c0 + q * L + y];
+            dugks_bgk(g, om_full);
+            __builtin_nontemporal_store(g[0], fp + c0 + y);
+        } else {
+#pragma unroll
+            for (int q = 1; q < 9; ++q) g[q] = fp[c0 + q * L + y];
+        }
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            const double v = dugks_face_update(
+                g[q], w[q][1][1], w[q][2][1], w[q][1][2], w[q][0][1], w[q][1][0], w[q][2][2],
+                w[q][0][2], w[q][0][0], w[q][2][0], dt * double(cx(q)), dt * double(cy(q)));
+            __builtin_nontemporal_store(v, fp + c0 + (size_t)q * L + y);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) w[q][0][j] = w[q][1][j], w[q][1][j] = w[q][2][j];
+        }
     }
 }
 
@@ -132,8 +170,30 @@ static int dugks_check(plbm_lattice *h, int a, int b) {
     return PLBM_OK;
 }
 
-static dim3 dugks_grid(const plbm_lattice *h) {
-    return dim3((unsigned)((h->ny + 255) / 256), (unsigned)h->nx);
+static int dugks_nby(const plbm_lattice *h) { return (h->ny + 255) / 256; }
+
+static int dugks_strip(const plbm_lattice *h) { return strip_columns(h->nx, h->ny); }
+
+template <bool FULL>
+static void launch_collide(plbm_lattice *h, const double *src, double *half, double *full,
+                           double om_full, double om_half) {
+    const int nby = dugks_nby(h);
+    hipLaunchKernelGGL(k_dugks_collide<FULL>, dim3((unsigned)(nby * h->nx)), dim3(256), 0,
+                       h->stream, src, half, full, h->ny, h->ldd, nby, om_full, om_half);
+}
+
+template <bool COLLIDE>
+static void launch_flux(plbm_lattice *h, const double *ft, const double *fo, double *fp,
+                        double dt, double om_full) {
+    const int xs = dugks_strip(h);
+    const dim3 grid((unsigned)dugks_nby(h), (unsigned)((h->nx + xs - 1) / xs));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, h->stream, ft, fo, fp, h->nx, h->ny,
+                           h->ldd, dt, om_full);
+    };
+    if (xs == 32) go(k_dugks_flux<COLLIDE, 32>);
+    else if (xs == 8) go(k_dugks_flux<COLLIDE, 8>);
+    else go(k_dugks_flux<COLLIDE, 1>);
 }
 
 }  // namespace plbm
@@ -145,9 +205,9 @@ extern "C" int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omeg
     PLBM_TRY(dugks_check(h, src, dst));
     PLBM_REQUIRE(std::isfinite(omega_full) && std::isfinite(omega_half), PLBM_ERR_ARG,
                  "non-finite relaxation rate");
-    hipLaunchKernelGGL(k_dugks_collide<true>, dugks_grid(h), dim3(256), 0, h->stream,
-                       static_cast<const double *>(h->f[src]), static_cast<double *>(h->f[src]),
-                       static_cast<double *>(h->f[dst]), h->ny, h->ldd, omega_full, omega_half);
+    launch_collide<true>(h, static_cast<const double *>(h->f[src]),
+                         static_cast<double *>(h->f[src]), static_cast<double *>(h->f[dst]),
+                         omega_full, omega_half);
     PLBM_HIP(hipGetLastError());
     return PLBM_OK;
 }
@@ -155,9 +215,8 @@ extern "C" int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omeg
 extern "C" int plbm_dugks_stream(plbm_lattice *h, int src, int dst, double dt) {
     PLBM_TRY(dugks_check(h, src, dst));
     PLBM_REQUIRE(std::isfinite(dt), PLBM_ERR_ARG, "non-finite dt");
-    hipLaunchKernelGGL(k_dugks_flux<false>, dugks_grid(h), dim3(256), 0, h->stream,
-                       static_cast<const double *>(h->f[src]), nullptr,
-                       static_cast<double *>(h->f[dst]), h->nx, h->ny, h->ldd, dt, 0.0);
+    launch_flux<false>(h, static_cast<const double *>(h->f[src]), nullptr,
+                       static_cast<double *>(h->f[dst]), dt, 0.0);
     PLBM_HIP(hipGetLastError());
     return PLBM_OK;
 }
@@ -170,12 +229,9 @@ extern "C" int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, 
                  PLBM_ERR_ARG, "non-finite dt or relaxation rate");
     double *latest = static_cast<double *>(h->f[iold]);
     double *half = static_cast<double *>(h->f[inew]);
-    const dim3 grid = dugks_grid(h);
     for (int s = 0; s < nsteps; ++s) {
-        hipLaunchKernelGGL(k_dugks_collide<false>, grid, dim3(256), 0, h->stream, latest, half,
-                           nullptr, h->ny, h->ldd, omega_full, omega_half);
-        hipLaunchKernelGGL(k_dugks_flux<true>, grid, dim3(256), 0, h->stream, half, latest,
-                           latest, h->nx, h->ny, h->ldd, dt, omega_full);
+        launch_collide<false>(h, latest, half, nullptr, omega_full, omega_half);
+        launch_flux<true>(h, half, latest, latest, dt, omega_full);
     }
     PLBM_HIP(hipGetLastError());
     if (nsteps % 2 == 1) std::swap(h->f[iold], h->f[inew]);

@@ -338,18 +338,34 @@ contains
    end subroutine
 
    !> Model id of the fused kernel for the grid's (streaming, collision) pair, or -1 when
-   !> the pair is not the engine's lbm_stream + collide_* (then the step is dispatched
-   !> through the procedure pointers).
+   !> the pair is not an engine streaming operator (lbm_stream or a FV/FD operator) with
+   !> collide_bgk/trt/rr (then the step is dispatched through the procedure pointers).
    integer function plbm_fused_model(grid) result(model)
       type(lattice_grid), intent(in) :: grid
       model = -1
-      if (.not. associated(grid%streaming, plbm_lbm_stream)) return
+      if (plbm_fused_op(grid) < 0) return
       if (associated(grid%collision, plbm_collide_bgk)) then
          model = PLBM_BGK
       else if (associated(grid%collision, plbm_collide_trt)) then
          model = PLBM_TRT
       else if (associated(grid%collision, plbm_collide_rr)) then
          model = PLBM_RR
+      end if
+   end function
+
+   !> Streaming-operator id (PLBM_STREAM_*) of grid%streaming, or -1 if it is not one of
+   !> the engine's streaming operators.
+   integer function plbm_fused_op(grid) result(op)
+      type(lattice_grid), intent(in) :: grid
+      op = -1
+      if (associated(grid%streaming, plbm_lbm_stream)) then
+         op = PLBM_STREAM_EXACT
+      else if (associated(grid%streaming, stream_fvm_bardow)) then
+         op = PLBM_STREAM_FVM_BARDOW
+      else if (associated(grid%streaming, stream_fdm_bardow)) then
+         op = PLBM_STREAM_FDM_BARDOW
+      else if (associated(grid%streaming, stream_fdm_sofonea)) then
+         op = PLBM_STREAM_FDM_SOFONEA
       end if
    end function
 
@@ -384,8 +400,8 @@ contains
    end subroutine
 
    !> One stream + collide step with the reference's buffer swap
-   !> (src/periodic_lbm.f90:15-29, src/fvm_bardow.F90:307-320).  The engine pair runs as
-   !> one fused kernel; other pairs go through the procedure pointers.
+   !> (src/periodic_lbm.f90:15-29, src/fvm_bardow.F90:307-320).  A pair of engine
+   !> operators runs as one fused kernel; other pairs go through the procedure pointers.
    subroutine perform_step(grid)
       type(lattice_grid), intent(inout) :: grid
       integer :: model, itmp
@@ -393,8 +409,9 @@ contains
       model = plbm_fused_model(grid)
       if (model >= 0) then
          call require_device(grid)
-         call plbm_check(plbm_step(grid%dev, int(model,c_int), int(grid%iold-1,c_int), &
-            int(grid%inew-1,c_int)), "perform_step")
+         call plbm_check(plbm_step_op(grid%dev, int(plbm_fused_op(grid),c_int), &
+            int(model,c_int), int(grid%iold-1,c_int), int(grid%inew-1,c_int), grid%dt), &
+            "perform_step")
          call device_written(grid)
       else
          call dispatch_streaming(grid)

@@ -141,6 +141,11 @@ class Lattice:
         """op: "exact" | "fvm_bardow" | "fdm_bardow" | "fdm_sofonea"."""
         check(lib().plbm_stream_op(self.h, _abi.STREAM_OPS.get(op, op), src, dst, dt))
 
+    def step_op(self, op: str | int, model: str | int, src: int, dst: int, dt: float = 1.0):
+        """Streaming operator `op` then collision `model`, fused: src -> dst."""
+        check(lib().plbm_step_op(self.h, _abi.STREAM_OPS.get(op, op),
+                                 _abi.MODEL_IDS.get(model, model), src, dst, dt))
+
     def collide(self, model: str | int, buf: int):
         check(lib().plbm_collide(self.h, _abi.MODEL_IDS.get(model, model), buf))
 
@@ -413,13 +418,19 @@ def _dispatch(grid: LatticeGrid, op, engine_ops):
     op(grid)
 
 
+_FUSABLE_STREAMING = {lbm_stream: "exact", stream_fvm_bardow: "fvm_bardow",
+                      stream_fdm_bardow: "fdm_bardow", stream_fdm_sofonea: "fdm_sofonea"}
+
+
 def perform_step(grid: LatticeGrid) -> None:
     """perform_step / perform_lbm_step (src/fvm_bardow.F90:307-320,
-    src/periodic_lbm.f90:15-29): the engine pair runs as one fused kernel."""
-    model = _ENGINE_COLLISIONS.get(grid.collision) if grid.streaming is lbm_stream else None
-    if model is not None:
+    src/periodic_lbm.f90:15-29): a pair of engine operators (lbm_stream or a FV/FD
+    streaming operator, and collide_bgk/trt/rr) runs as one fused kernel."""
+    model = _ENGINE_COLLISIONS.get(grid.collision)
+    op = _FUSABLE_STREAMING.get(grid.streaming)
+    if model is not None and op is not None:
         _require_device(grid)
-        grid.lattice.step(model, grid.iold - 1, grid.inew - 1)
+        grid.lattice.step_op(op, model, grid.iold - 1, grid.inew - 1, grid.dt)
         _device_written(grid)
     else:
         _dispatch(grid, grid.streaming, _ENGINE_STREAMING)

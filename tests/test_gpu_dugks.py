@@ -30,7 +30,8 @@ def rates(nu, dt):
     return dt / (tau + 0.5 * dt), 1.0 / (tau / dt + 0.5)
 
 
-@pytest.mark.parametrize("nx,ny,dt", [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 2.0), (300, 257, 0.8)])
+@pytest.mark.parametrize("nx,ny,dt", [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 2.0), (300, 257, 0.8),
+                                      (1100, 1000, 0.7), (3300, 2500, 1.0)])
 def test_dugks_operators_bit_exact(P, gpu, oracle, nx, ny, dt):
     """dugks_collide then dugks_stream, launched one by one, 5 steps."""
     print(f"case dugks operators {nx}x{ny} dt={dt}", flush=True)
@@ -51,8 +52,11 @@ def test_dugks_operators_bit_exact(P, gpu, oracle, nx, ny, dt):
     np.testing.assert_array_equal(lat.download_f(0)[:ny], b[:ny])
 
 
+# (1100, 1000) and (3300, 2500) run the flux kernel with 8- and 32-column strips whose
+# last strip is ragged; the small grids use single-column strips.
 @pytest.mark.parametrize("nsteps", [1, 2, 7])
-@pytest.mark.parametrize("nx,ny,dt", [(64, 64, 1.0), (45, 130, 0.6)])
+@pytest.mark.parametrize("nx,ny,dt", [(64, 64, 1.0), (45, 130, 0.6), (1100, 1000, 0.9),
+                                      (3300, 2500, 1.0)])
 def test_dugks_fused_step_slots(P, gpu, oracle, nx, ny, dt, nsteps):
     """plbm_dugks_step: after n steps each slot holds what the reference's buffer of the
     same index holds (latest populations and the half-collided state)."""

@@ -24,8 +24,13 @@ def fields(nx, ny, seed):
             np.asfortranarray(0.05 * rng.standard_normal((ny, nx))))
 
 
+# (1100, 1000) and (3300, 2500) run the stencil kernels with 8- and 32-column strips whose
+# last strip is ragged; the small grids use single-column strips.
+SIZES = [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 0.8), (1100, 1000, 0.7), (3300, 2500, 1.0)]
+
+
 @pytest.mark.parametrize("op", OPS)
-@pytest.mark.parametrize("nx,ny,dt", [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 0.8)])
+@pytest.mark.parametrize("nx,ny,dt", SIZES)
 def test_streaming_operator_bit_exact(P, gpu, oracle, op, nx, ny, dt):
     print(f"case {op} {nx}x{ny} dt={dt}", flush=True)
     rho, ux, uy = fields(nx, ny, nx + ny)
@@ -43,6 +48,50 @@ def test_streaming_operator_bit_exact(P, gpu, oracle, op, nx, ny, dt):
         oracle.collide("bgk", nx, ny, b, omega)
         a, b = b, a
     np.testing.assert_array_equal(lat.download_f(0)[:ny], a[:ny])
+
+
+@pytest.mark.parametrize("model", ["bgk", "trt", "rr"])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("nx,ny,dt", [SIZES[0], SIZES[2], SIZES[3], SIZES[4]])
+def test_fused_operator_step_bit_exact(P, gpu, oracle, op, model, nx, ny, dt):
+    """plbm_step_op (stream operator + collision in one pass) against the oracle's
+    stream_op then collide, 3 steps, FP64."""
+    print(f"case fused {op}+{model} {nx}x{ny} dt={dt}", flush=True)
+    rho, ux, uy = fields(nx, ny, nx + 2 * ny)
+    omega = 1.0 / (3 * 0.01 / dt + 0.5)
+    lat = P.Lattice(nx, ny, 2, "f64")
+    lat.set_params(omega, 0.25)
+    lat.equilibrium_init(0, rho, ux, uy)
+    a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+    for s in range(3):
+        lat.step_op(op, model, s % 2, 1 - s % 2, dt)
+        oracle.stream_op(op, nx, ny, a, b, dt)
+        oracle.collide(model, nx, ny, b, omega, 0.25)
+        a, b = b, a
+    np.testing.assert_array_equal(lat.download_f(1)[:ny], a[:ny])
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_fused_operator_step_shifted_storage(P, gpu, op):
+    """FP32-shifted storage: the fused pass skips the intermediate FP32 rounding of the
+    unfused pair, so both agree to FP32 roundoff."""
+    nx, ny = 300, 200
+    rho, ux, uy = fields(nx, ny, 5)
+    out = []
+    for fused in (True, False):
+        lat = P.Lattice(nx, ny, 2, "f32s")
+        lat.set_params(1.7, 0.25)
+        lat.equilibrium_init(0, rho, ux, uy)
+        for s in range(10):
+            if fused:
+                lat.step_op(op, "trt", s % 2, 1 - s % 2, 0.9)
+            else:
+                lat.stream_op(op, s % 2, 1 - s % 2, 0.9)
+                lat.collide("trt", 1 - s % 2)
+        out.append(lat.macros(0))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=2e-6)
 
 
 @pytest.mark.parametrize("op", OPS)
@@ -64,8 +113,10 @@ def test_streaming_operator_shifted_storage(P, gpu, oracle, op):
         assert err <= tol, (storage, err)
 
 
-def test_reference_named_api_with_fvm_streaming(P, gpu, oracle):
-    """perform_step with grid.streaming = stream_fvm_bardow (unfused device path)."""
+@pytest.mark.parametrize("fused", [True, False])
+def test_reference_named_api_with_fvm_streaming(P, gpu, oracle, fused):
+    """perform_step with grid.streaming = stream_fvm_bardow (the fused device pass), and
+    the same pair called one by one through the slots."""
     n, steps = 32, 12
     vc = P.VortexCase(0.05, n / 2, n / 2, n / 10, 0.1)
     rho, ux, uy = vc.eval(n, n)
@@ -75,7 +126,12 @@ def test_reference_named_api_with_fvm_streaming(P, gpu, oracle):
     g.streaming, g.collision = P.stream_fvm_bardow, P.collide_bgk
     P.set_pdf_to_equilibrium(g)
     for _ in range(steps):
-        P.perform_step(g)
+        if fused:
+            P.perform_step(g)
+        else:
+            g.streaming(g)
+            g.collision(g)
+            g.iold, g.inew = g.inew, g.iold
     a, b = po.new_f(n, n), po.new_f(n, n)
     oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, a)
     for _ in range(steps):
