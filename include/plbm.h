@@ -43,6 +43,11 @@ extern "C" {
 #define PLBM_BGK 0
 #define PLBM_TRT 1
 #define PLBM_RR 2
+/* collide_bgk as the reference builds it with -DSPLIT (bgk_kernel_cache,
+ * src/collision_bgk.F90:84-176): the same BGK model in split-weight algebra, bit-identical
+ * to that build in F64 storage (it differs from PLBM_BGK at roundoff level); the shifted
+ * storages run the shifted BGK for both. */
+#define PLBM_BGK_SPLIT 3
 
 /* Population storage.  F64 reproduces the reference arithmetic bit for bit.  The
  * shifted forms store g_q = f_q - w_q*rho0 (rho0 = 1): F64S in FP64 with FP64 math,

@@ -196,7 +196,8 @@ void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *
  * invariants: omega = 1 in the collision gives the SPLIT-form equilibrium, the stream
  * conserves mass and momentum on a periodic grid, dt = 0 leaves fp unchanged. */
 
-/* kernel_bgk, src/periodic_dugks.F90:80-165: BGK in the SPLIT algebra on one cell. */
+/* kernel_bgk, src/periodic_dugks.F90:80-165 (identical to bgk_kernel_cache,
+ * src/collision_bgk.F90:103-170): BGK in the split-weight algebra on one cell. */
 static void dugks_bgk_cell(double f[9], double omega) {
     const double omegabar = 1.0 - omega;
     const double om_w0 = (3.0 * omega) * (4.0 / 9.0), om_ws = (3.0 * omega) * (1.0 / 9.0),
@@ -220,6 +221,20 @@ static void dugks_bgk_cell(double f[9], double omega) {
     const double xmy = ux - uy, t68 = indp + (1.5 * xmy) * xmy;
     f[6] = omegabar * f[6] + (om_wd * rho) * (t68 - xmy);
     f[8] = omegabar * f[8] + (om_wd * rho) * (t68 + xmy);
+}
+
+/* bgk_kernel_cache, src/collision_bgk.F90:84-176 (collide_bgk of the -DSPLIT build): the
+ * same split-weight algebra as DUGKS's kernel_bgk.  Pinned bit for bit against the
+ * reference compiled with -DSPLIT (oracle/Makefile target ref-split). */
+void orc_collide_bgk_split(int nx, int ny, int ld, double *f, double omega) {
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            double c[9];
+            for (int q = 0; q < 9; ++q) c[q] = f[IDX(q, x, y)];
+            dugks_bgk_cell(c, omega);
+            for (int q = 0; q < 9; ++q) f[IDX(q, x, y)] = c[q];
+        }
 }
 
 /* dugks_collide, src/periodic_dugks.F90:46-77: fnew = BGK(fold, omega_full) (the copy
@@ -446,6 +461,7 @@ void orc_collide(int model, int nx, int ny, int ld, double *f, double omega, dou
     case 0: orc_collide_bgk(nx, ny, ld, f, omega); break;
     case 1: orc_collide_trt(nx, ny, ld, f, omega, orc_lambda_d(omega, trt_magic)); break;
     case 2: orc_collide_rr(nx, ny, ld, f, omega); break;
+    case 3: orc_collide_bgk_split(nx, ny, ld, f, omega); break;
     default: abort();
     }
 }

@@ -43,9 +43,11 @@ void orc_dugks_collide(int nx, int ny, int ld, double *fold, double *fnew, doubl
                        double omega_half);
 void orc_dugks_stream(int nx, int ny, int ld, const double *ft, double *fp, double dt);
 void orc_collide_bgk(int nx, int ny, int ld, double *f, double omega);
+/* collide_bgk of the -DSPLIT build (bgk_kernel_cache, src/collision_bgk.F90:84-176) */
+void orc_collide_bgk_split(int nx, int ny, int ld, double *f, double omega);
 void orc_collide_trt(int nx, int ny, int ld, double *f, double lambda_e, double lambda_d);
 void orc_collide_rr(int nx, int ny, int ld, double *f, double omega);
-/* model: 0 BGK, 1 TRT, 2 RR.  trt_magic feeds lambda_d() as collide_trt does. */
+/* model: 0 BGK, 1 TRT, 2 RR, 3 BGK (-DSPLIT algebra).  trt_magic feeds lambda_d() as collide_trt does. */
 void orc_collide(int model, int nx, int ny, int ld, double *f, double omega, double trt_magic);
 void orc_update_macros(int nx, int ny, int ld, const double *f, double *rho, double *ux,
                        double *uy);

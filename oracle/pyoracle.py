@@ -23,6 +23,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIBORACLE = HERE / "liboracle.so"
 LIBREF = HERE / "_ref" / "libplbm_ref.so"
+LIBREF_SPLIT = HERE / "_ref" / "split" / "libplbm_ref.so"  # built with -DSPLIT
 REF_APP = HERE / "_ref" / "main_taylor_green_ref"
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -30,7 +31,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _i = ctypes.c_int
 _d = ctypes.c_double
 
-MODELS = {"bgk": 0, "trt": 1, "rr": 2}
+MODELS = {"bgk": 0, "trt": 1, "rr": 2, "bgk_split": 3}
 
 
 def _ptr(a: np.ndarray):

@@ -52,6 +52,9 @@ contains
          grid%collision => collide_trt
       case (2)
          grid%collision => collide_rr
+      case (3)
+         ! collide_bgk as compiled: bgk_kernel_cache in the -DSPLIT build (ref-split)
+         grid%collision => collide_bgk
       case default
          error stop "ref_driver: unknown collision model"
       end select

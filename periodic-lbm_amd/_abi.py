@@ -17,12 +17,12 @@ LIBPATH = PKG / "libplbm.so"
 # status codes and enums, include/plbm.h
 OK = 0
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_NUMERIC, ERR_TIMEOUT = 1, 2, 3, 4, 5, 6
-BGK, TRT, RR = 0, 1, 2
+BGK, TRT, RR, BGK_SPLIT = 0, 1, 2, 3
 F64, F64S, F32S, F16S = 0, 1, 2, 3
 RED_MASS, RED_KE, RED_ERR2, RED_REF2, RED_UMAX, RED_NONFINITE, RED_U2 = range(7)
 NRED = 8
 
-MODEL_IDS = {"bgk": BGK, "trt": TRT, "rr": RR}
+MODEL_IDS = {"bgk": BGK, "trt": TRT, "rr": RR, "bgk_split": BGK_SPLIT}
 STORAGE_IDS = {"f64": F64, "f64s": F64S, "f32s": F32S, "f16s": F16S}
 STORAGE_BYTES = {F64: 8, F64S: 8, F32S: 4, F16S: 2}
 

@@ -44,7 +44,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-Wa
 # Fortran shim modules in dependency order
 FORTRAN_SOURCES = [
     "plbm_capi.f90", "precision.F90", "fvm_bardow.F90", "periodic_lbm.f90",
-    "collision_bgk.f90", "collision_trt.f90", "collision_regularized.f90",
+    "collision_bgk.F90", "collision_trt.f90", "collision_regularized.f90",
     "taylor_green.f90", "shear_layer.f90", "barotropic_vortex_case.f90", "periodic_dugks.f90",
 ]
 
@@ -75,6 +75,12 @@ def build_hip(verbose: bool = True) -> Path:
     return LIB
 
 
+# Preprocessor options of the reference's Fortran build that the shim honours, e.g.
+# PLBM_FORTRAN_DEFINES="-DSPLIT" (collide_bgk in split-weight algebra, as the reference's
+# src/collision_bgk.F90:23 selects it).
+FORTRAN_DEFINES = os.environ.get("PLBM_FORTRAN_DEFINES", "").split()
+
+
 def build_fortran(verbose: bool = True) -> Path:
     """Compile the Fortran module shim (ISO_C_BINDING over libplbm.so)."""
     mod = FBUILD / "mod"
@@ -82,8 +88,8 @@ def build_fortran(verbose: bool = True) -> Path:
     objs = []
     for s in FORTRAN_SOURCES:
         o = FBUILD / (Path(s).stem + ".o")
-        _run([FLANG, "-O2", "-fPIC", "-fopenmp", "-module-dir", mod, f"-I{mod}", "-c",
-              FSRC / s, "-o", o])
+        _run([FLANG, "-O2", "-fPIC", "-fopenmp", *FORTRAN_DEFINES, "-module-dir", mod,
+              f"-I{mod}", "-c", FSRC / s, "-o", o])
         objs.append(o)
     lib = FBUILD / "libplbm_fortran.a"
     if lib.exists():

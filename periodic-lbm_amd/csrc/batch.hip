@@ -405,7 +405,7 @@ int plbm_batch_create(int ncases, const int *nx, const int *ny, int storage, int
     *out = nullptr;
     PLBM_REQUIRE(ncases >= 1 && nx && ny, PLBM_ERR_ARG, "need ncases >= 1 and size arrays");
     PLBM_REQUIRE(storage >= PLBM_F64 && storage <= PLBM_F16S, PLBM_ERR_ARG, "unknown storage");
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     auto *b = new plbm_batch();
     b->ncases = ncases;
     b->storage = storage;

@@ -126,12 +126,14 @@ inline int dispatch_model(int m, F &&f) {
     case PLBM_BGK: return f(std::integral_constant<int, PLBM_BGK>{});
     case PLBM_TRT: return f(std::integral_constant<int, PLBM_TRT>{});
     case PLBM_RR: return f(std::integral_constant<int, PLBM_RR>{});
+    case PLBM_BGK_SPLIT: return f(std::integral_constant<int, PLBM_BGK_SPLIT>{});
     }
     set_error("unknown collision model " + std::to_string(m));
     return PLBM_ERR_ARG;
 }
 
 constexpr int kMaxBuffers = 4;
+constexpr int kModels = 4;  // PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT
 constexpr int kReduceBlocks = 1024;
 
 }  // namespace plbm
@@ -145,8 +147,8 @@ struct plbm_lattice {
     double omega = 0, trt_magic = 0, lambda_odd = 0;
     bool params_set = false;
     double scale = 1.0;            // FP16 storage scale
-    plbm::LaunchCfg cfg[3];        // fused-step launch shape per collision model
-    bool tuned[3] = {false, false, false};
+    plbm::LaunchCfg cfg[plbm::kModels];  // fused-step launch shape per collision model
+    bool tuned[plbm::kModels] = {};
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

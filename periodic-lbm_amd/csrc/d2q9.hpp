@@ -72,6 +72,36 @@ __device__ inline void collide_bgk_ref(double f[9], double omega) {
     for (int q = 0; q < 9; ++q) f[q] = omegabar * f[q] + omega * feq[q];
 }
 
+// bgk_kernel_cache body, src/collision_bgk.F90:103-170 (the -DSPLIT build of collide_bgk):
+// the same BGK model in split-weight algebra, f_q <- (1-omega) f_q + 3 omega w_q rho
+// (1/3 - u^2/2 + 1.5 (c_q.u)^2 + c_q.u).  kernel_bgk of src/periodic_dugks.F90:80-165 is
+// the same code.
+__device__ inline void collide_bgk_split_ref(double f[9], double omega) {
+#pragma clang fp contract(off)
+    const double omegabar = 1.0 - omega;
+    const double om_w0 = (3.0 * omega) * (4.0 / 9.0), om_ws = (3.0 * omega) * (1.0 / 9.0),
+                 om_wd = (3.0 * omega) * (1.0 / 36.0);
+    const double rho =
+        (((f[5] + f[7]) + (f[6] + f[8])) + ((f[1] + f[3]) + (f[2] + f[4]))) + f[0];
+    const double invrho = 1.0 / rho;
+    const double ux = invrho * (((f[5] - f[7]) + (f[8] - f[6])) + (f[1] - f[3]));
+    const double uy = invrho * (((f[5] - f[7]) + (f[6] - f[8])) + (f[2] - f[4]));
+    const double indp = 1.0 / 3.0 - 0.5 * (ux * ux + uy * uy);
+    f[0] = omegabar * f[0] + (om_w0 * rho) * indp;
+    const double t13 = indp + (1.5 * ux) * ux;
+    f[1] = omegabar * f[1] + (om_ws * rho) * (t13 + ux);
+    f[3] = omegabar * f[3] + (om_ws * rho) * (t13 - ux);
+    const double t24 = indp + (1.5 * uy) * uy;
+    f[2] = omegabar * f[2] + (om_ws * rho) * (t24 + uy);
+    f[4] = omegabar * f[4] + (om_ws * rho) * (t24 - uy);
+    const double xpy = ux + uy, t57 = indp + (1.5 * xpy) * xpy;
+    f[5] = omegabar * f[5] + (om_wd * rho) * (t57 + xpy);
+    f[7] = omegabar * f[7] + (om_wd * rho) * (t57 - xpy);
+    const double xmy = ux - uy, t68 = indp + (1.5 * xmy) * xmy;
+    f[6] = omegabar * f[6] + (om_wd * rho) * (t68 - xmy);
+    f[8] = omegabar * f[8] + (om_wd * rho) * (t68 + xmy);
+}
+
 // trt_naive body, src/collision_trt.F90:97-152 (incompressible: j is not divided by rho)
 __device__ inline void collide_trt_ref(double f[9], double lambda_e, double lambda_d) {
 #pragma clang fp contract(off)

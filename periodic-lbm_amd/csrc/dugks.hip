@@ -30,31 +30,9 @@
 
 namespace plbm {
 
-// kernel_bgk (src/periodic_dugks.F90:80-165): BGK in the split-weight algebra.
+// kernel_bgk (src/periodic_dugks.F90:80-165) is collide_bgk's split-weight algebra.
 __device__ __forceinline__ void dugks_bgk(double f[9], double omega) {
-#pragma clang fp contract(off)
-    const double omegabar = 1.0 - omega;
-    const double om_w0 = (3.0 * omega) * (4.0 / 9.0), om_ws = (3.0 * omega) * (1.0 / 9.0),
-                 om_wd = (3.0 * omega) * (1.0 / 36.0);
-    const double rho =
-        (((f[5] + f[7]) + (f[6] + f[8])) + ((f[1] + f[3]) + (f[2] + f[4]))) + f[0];
-    const double invrho = 1.0 / rho;
-    const double ux = invrho * (((f[5] - f[7]) + (f[8] - f[6])) + (f[1] - f[3]));
-    const double uy = invrho * (((f[5] - f[7]) + (f[6] - f[8])) + (f[2] - f[4]));
-    const double indp = 1.0 / 3.0 - 0.5 * (ux * ux + uy * uy);
-    f[0] = omegabar * f[0] + (om_w0 * rho) * indp;
-    const double t13 = indp + (1.5 * ux) * ux;
-    f[1] = omegabar * f[1] + (om_ws * rho) * (t13 + ux);
-    f[3] = omegabar * f[3] + (om_ws * rho) * (t13 - ux);
-    const double t24 = indp + (1.5 * uy) * uy;
-    f[2] = omegabar * f[2] + (om_ws * rho) * (t24 + uy);
-    f[4] = omegabar * f[4] + (om_ws * rho) * (t24 - uy);
-    const double xpy = ux + uy, t57 = indp + (1.5 * xpy) * xpy;
-    f[5] = omegabar * f[5] + (om_wd * rho) * (t57 + xpy);
-    f[7] = omegabar * f[7] + (om_wd * rho) * (t57 - xpy);
-    const double xmy = ux - uy, t68 = indp + (1.5 * xmy) * xmy;
-    f[6] = omegabar * f[6] + (om_wd * rho) * (t68 - xmy);
-    f[8] = omegabar * f[8] + (om_wd * rho) * (t68 + xmy);
+    collide_bgk_split_ref(f, omega);
 }
 
 // One thread per cell on a 1-D grid of (ny/256) x nx workgroups, renumbered so that each

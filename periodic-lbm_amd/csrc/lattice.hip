@@ -320,7 +320,7 @@ int launch_step_cfg(plbm_lattice *h, int model, const void *src, void *dst, Laun
 }
 
 int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG,
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
     return launch_step_cfg(h, model, src, dst, h->cfg[model]);
 }
@@ -658,7 +658,7 @@ int plbm_collide(plbm_lattice *h, int model, int buf) {
     PLBM_TRY(check_buf(h, buf));
     PLBM_TRY(require_params(h));
     StepArgs a = make_args(h, h->f[buf], h->f[buf]);
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG,
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
     return dispatch_model(model, [&](auto M) {
         return dispatch_storage(h->storage, [&](auto S) {
@@ -750,7 +750,7 @@ int plbm_sync(plbm_lattice *h) {
 
 int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     LaunchCfg c{block, xcd_remap};
     PLBM_REQUIRE(valid_cfg(c), PLBM_ERR_ARG,
                  "block must be a multiple of 64 in [64,1024] and xcd_remap 0 or 1");
@@ -760,7 +760,7 @@ int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
 
 int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap) {
     PLBM_REQUIRE(h && block && xcd_remap, PLBM_ERR_ARG, "null argument");
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     *block = h->cfg[model].block;
     *xcd_remap = h->cfg[model].remap;
     return PLBM_OK;
@@ -769,7 +769,7 @@ int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap
 int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms) {
     PLBM_TRY(check_buf(h, buf));
     PLBM_TRY(require_params(h));
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     PLBM_REQUIRE(reps >= 1, PLBM_ERR_ARG, "reps must be >= 1");
     LaunchCfg best;
     double ms = 0;

@@ -48,6 +48,7 @@ __device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const 
     using R = typename Traits<ST>::R;
     if constexpr (!Traits<ST>::shifted) {
         if constexpr (MODEL == PLBM_BGK) collide_bgk_ref(f, a.omega);
+        else if constexpr (MODEL == PLBM_BGK_SPLIT) collide_bgk_split_ref(f, a.omega);
         else if constexpr (MODEL == PLBM_TRT) collide_trt_ref(f, a.omega, a.lambda_odd);
         else collide_rr_ref(f, a.omega);
     } else {
@@ -55,7 +56,7 @@ __device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const 
         const R lo = sizeof(R) == 8 ? R(a.lambda_odd) : R(a.lambda_odd_f);
         R d0, jx0, jy0;
         if constexpr (sizeof(R) == 4) moments_shifted(f, d0, jx0, jy0);
-        if constexpr (MODEL == PLBM_BGK) collide_bgk_shifted<R>(f, om);
+        if constexpr (MODEL == PLBM_BGK || MODEL == PLBM_BGK_SPLIT) collide_bgk_shifted<R>(f, om);
         else if constexpr (MODEL == PLBM_TRT) collide_trt_shifted<R>(f, om, lo);
         else collide_rr_shifted<R>(f, om);
         if constexpr (sizeof(R) == 4) restore_conservation(f, d0, jx0, jy0);

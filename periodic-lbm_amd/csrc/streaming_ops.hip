@@ -181,6 +181,7 @@ int launch_fd(plbm_lattice *h, int op, int model, int src, int dst, double dt) {
             case PLBM_BGK: return go(O, std::integral_constant<int, PLBM_BGK>{});
             case PLBM_TRT: return go(O, std::integral_constant<int, PLBM_TRT>{});
             case PLBM_RR: return go(O, std::integral_constant<int, PLBM_RR>{});
+            case PLBM_BGK_SPLIT: return go(O, std::integral_constant<int, PLBM_BGK_SPLIT>{});
             default: return go(O, std::integral_constant<int, -1>{});
             }
         };
@@ -219,7 +220,7 @@ extern "C" int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double 
 
 extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt) {
     PLBM_TRY(check_op_args(h, op, src, dst, dt));
-    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_RR, PLBM_ERR_ARG,
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
     PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE, "plbm_set_params has not been called");
     if (op == PLBM_STREAM_EXACT) return plbm_step(h, model, src, dst);

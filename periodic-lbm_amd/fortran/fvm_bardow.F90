@@ -39,10 +39,11 @@ module fvm_bardow
    ! engine extensions (prefixed so they cannot clash with program names)
    public :: plbm_sync_host, plbm_sync_device, plbm_fused_model
    public :: plbm_lbm_stream, plbm_collide_bgk, plbm_collide_trt, plbm_collide_rr
+   public :: plbm_collide_bgk_split
    public :: plbm_reductions
    public :: plbm_grid_dugks_collide, plbm_grid_dugks_stream, plbm_grid_perform_dugks_step
    public :: plbm_tau_over_dt
-   public :: PLBM_F64, PLBM_F64S, PLBM_F32S, PLBM_F16S
+   public :: PLBM_F64, PLBM_F64S, PLBM_F32S, PLBM_F16S, PLBM_BGK_SPLIT
 
    integer, parameter :: WHERE_HOST = 0, WHERE_DEVICE = 1
 
@@ -325,6 +326,13 @@ contains
       call collide_on_device(grid, PLBM_BGK, "collide_bgk")
    end subroutine
 
+   !> collide_bgk of the -DSPLIT build (bgk_kernel_cache, src/collision_bgk.F90:84-176)
+   !> on the device, in place on f(inew).
+   subroutine plbm_collide_bgk_split(grid)
+      class(lattice_grid), intent(inout) :: grid
+      call collide_on_device(grid, PLBM_BGK_SPLIT, "collide_bgk")
+   end subroutine
+
    !> collide_trt (src/collision_trt.F90:41-160) on the device, in place on f(inew).
    subroutine plbm_collide_trt(grid)
       class(lattice_grid), intent(inout) :: grid
@@ -350,6 +358,8 @@ contains
          model = PLBM_TRT
       else if (associated(grid%collision, plbm_collide_rr)) then
          model = PLBM_RR
+      else if (associated(grid%collision, plbm_collide_bgk_split)) then
+         model = PLBM_BGK_SPLIT
       end if
    end function
 
@@ -383,6 +393,7 @@ contains
       yes = associated(grid%collision, plbm_collide_bgk) .or. &
             associated(grid%collision, plbm_collide_trt) .or. &
             associated(grid%collision, plbm_collide_rr) .or. &
+            associated(grid%collision, plbm_collide_bgk_split) .or. &
             associated(grid%collision, plbm_grid_dugks_collide)
    end function
 

@@ -8,7 +8,8 @@ module plbm_capi
    private
 
    integer(c_int), parameter, public :: PLBM_OK = 0
-   integer(c_int), parameter, public :: PLBM_BGK = 0, PLBM_TRT = 1, PLBM_RR = 2
+   integer(c_int), parameter, public :: PLBM_BGK = 0, PLBM_TRT = 1, PLBM_RR = 2, &
+                                        PLBM_BGK_SPLIT = 3
    integer(c_int), parameter, public :: PLBM_F64 = 0, PLBM_F64S = 1, PLBM_F32S = 2, PLBM_F16S = 3
    integer(c_int), parameter, public :: PLBM_NRED = 8
 

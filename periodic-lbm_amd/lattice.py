@@ -357,6 +357,12 @@ def collide_bgk(grid: LatticeGrid) -> None:
     _collide(grid, _abi.BGK)
 
 
+def collide_bgk_split(grid: LatticeGrid) -> None:
+    """collide_bgk as the reference's -DSPLIT build runs it (bgk_kernel_cache,
+    src/collision_bgk.F90:84-176), in place on f(inew)."""
+    _collide(grid, _abi.BGK_SPLIT)
+
+
 def collide_trt(grid: LatticeGrid) -> None:
     """collide_trt (src/collision_trt.F90:41-160), in place on f(inew)."""
     _collide(grid, _abi.TRT)
@@ -407,7 +413,7 @@ def perform_dugks_step(grid: LatticeGrid) -> None:
 
 
 _ENGINE_COLLISIONS = {collide_bgk: _abi.BGK, collide_trt: _abi.TRT, collide_rr: _abi.RR,
-                      dugks_collide: None}
+                      collide_bgk_split: _abi.BGK_SPLIT, dugks_collide: None}
 _ENGINE_STREAMING = (lbm_stream, stream_fvm_bardow, stream_fdm_bardow, stream_fdm_sofonea,
                      dugks_stream)
 

@@ -16,6 +16,8 @@ data files in this directory:
   ref_dugks.npz   DUGKS (src/periodic_dugks.F90): both buffers after 1 and 7 steps and
                   the update_macros fields, square and non-square, dt = 1 and 0.5
   ref_dugks_app.json  sha256 of the raw fields tests/fortran/dugks_check.f90 writes
+  ref_split.npz   the reference built with -DSPLIT (oracle/_ref/split): collide_bgk
+                  (bgk_kernel_cache) and collide_trt (trt_split), 1/10/100 steps
 
 Usage (in a container that has the reference tree and flang):
   make -C oracle ref && python tests/golden/make_golden.py [--app]
@@ -224,6 +226,33 @@ def dugks_app():
                                                         indent=1))
 
 
+def split():
+    """perform_lbm_step with the -DSPLIT build's collide_bgk / collide_trt."""
+    RS = po.RefLib(po.LIBREF_SPLIT)
+    out = {}
+    for nx, ny in [(16, 16), (12, 20)]:
+        rng = np.random.default_rng(nx * 5 + ny)
+        rho = np.asfortranarray(1.0 + 0.01 * rng.standard_normal((ny, nx)))
+        ux = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        uy = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        key = f"{nx}x{ny}"
+        out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+        for m in ("bgk", "trt"):
+            g = RS.grid(nx, ny, m)
+            g.set_properties(0.01, 1.0, 0.25)
+            g.set_macros(rho, ux, uy)
+            g.set_pdf_to_equilibrium()
+            out[f"{key}_omega"] = np.array([g.params()["omega"]])
+            done = 0
+            for n in (1, 10, 100):
+                g.step(n - done)
+                done = n
+                iold, _, _, _ = g.indices()
+                out[f"{key}_{m}_{n}_latest"] = g.get_f(iold)[:ny]
+            g.close()
+    np.savez_compressed(HERE / "ref_split.npz", **out)
+
+
 def outputs():
     """sha256 of the files tests/fortran/outputs_check.f90 writes through the reference's
     output_vtk / output_npy / output_gnuplot (oracle/_ref/outputs_check_ref)."""
@@ -237,6 +266,9 @@ def outputs():
 
 if __name__ == "__main__":
     R = po.RefLib()
+    if "--split-only" in sys.argv:
+        split()
+        sys.exit(0)
     if "--dugks-only" in sys.argv:
         dugks(R)
         dugks_app()
@@ -252,6 +284,7 @@ if __name__ == "__main__":
         streaming_ops(R)
         dugks(R)
         dugks_app()
+        split()
         if "--app" in sys.argv:
             app()
             vortex_app()
