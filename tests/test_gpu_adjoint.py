@@ -73,7 +73,7 @@ def test_autograd_function(P, gpu):
     rho0 = torch.ones_like(ux0)
     l2 = eng2.forward(rho0, ux0, uy0, 1.0 / TAU, uxt, uyt)
     _, gux, guy, gom = eng2.backward()
-    assert float(loss) == l2
+    assert float(loss.detach()) == l2
     torch.testing.assert_close(ux.grad, gux, rtol=0, atol=0)
     torch.testing.assert_close(uy.grad, guy, rtol=0, atol=0)
     assert float(tau.grad) == pytest.approx(-gom / TAU ** 2, rel=1e-15)
