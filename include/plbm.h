@@ -123,7 +123,9 @@ int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const
 int plbm_step(plbm_lattice *h, int model, int src, int dst);
 /* perform_lbm_step (src/periodic_lbm.f90:15-29) nsteps times with the reference's
  * buffer bookkeeping: on entry *iold / *inew are the current ids; each step reads *iold,
- * writes *inew, then swaps them.  Asynchronous. */
+ * writes *inew, then swaps them.  Runs in passes of several steps where that is faster
+ * (plbm_set_steps_per_launch); the slot contents afterwards are the reference's.
+ * Asynchronous. */
 int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew);
 /* Unfused operators for the procedure-pointer slots grid%streaming / grid%collision:
  * lbm_stream (src/periodic_lbm.f90:32-129) src -> dst; collide_* in place on buf. */
@@ -216,10 +218,27 @@ double plbm_magic_number(double le, double ld);
  * [64, 1024]) and XCD-contiguous workgroup renumbering (0/1).  Default 256 / 1. */
 int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap);
 int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap);
+/* Temporal blocking of plbm_run: steps advanced per pass over HBM (1 = the single-step
+ * kernel; 2..4 for F64 / F64S storage or 2..8 for F32S / F16S = that many fused steps
+ * per launch, each wavefront walking a strip of columns with the intermediate states in
+ * registers; 0 = chosen from the grid size: 4 when the grid gives at least 2048
+ * wavefronts, else 1).  In F64 storage the results are bit-identical for every setting;
+ * the shifted storages agree to roundoff (their collisions allow FMA contraction, which
+ * the compiler places per kernel).  plbm_run keeps the reference's slot contents (latest
+ * in *iold, the state one step earlier in *inew); with passes of more than one step it
+ * may exchange the device storage of the two slots, so addresses from plbm_buffer_ptr
+ * must be fetched again after it.  The get variant reports the effective value. */
+int plbm_set_steps_per_launch(plbm_lattice *h, int model, int steps);
+/* One pass of `steps` (1..4, or 1..8 for F32S / F16S) fused steps from buffer src to
+ * buffer dst, without slot bookkeeping and without writing the intermediate states:
+ * the building block of plbm_run.  Asynchronous. */
+int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst);
+int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *steps);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
- * buffer buf into a scratch buffer; the lattice's buffers are not modified), keep the
- * fastest for this lattice and cache it per (device, storage, model, grid) for the
- * process.  *best_ms (may be NULL) receives the winner's time per step. */
+ * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
+ * of 2..4 (2..8) steps against it per lattice update; keep the fastest for this lattice and
+ * cache it per (device, storage, model, grid) for the process.  *best_ms (may be NULL)
+ * receives the winner's time per lattice-update step. */
 int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms);
 
 /* ---------------------------------------------------------------- batched engine */

@@ -29,7 +29,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip", "dugks.hip"]
+HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip", "dugks.hip",
+               "multistep.hip"]
 CXX_SOURCES = ["fields.cpp", "sim_plugin.cpp"]
 # -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
 # reference is built without FMA); kernels that want FMA enable it locally by pragma.

@@ -82,6 +82,8 @@ __host__ __device__ inline size_t pop_index(int q, int x, int y, int ldd) {
 struct LaunchCfg {
     int block = 256;
     int remap = 1;
+    int ns = 0;  // steps per pass of plbm_run (1 = single-step kernel, 2..8 = temporal
+                 // blocking, 0 = chosen from the grid size)
 };
 
 // Per-launch scalars of one step, passed by value.
@@ -133,7 +135,9 @@ inline int dispatch_model(int m, F &&f) {
 }
 
 constexpr int kMaxBuffers = 4;
-constexpr int kModels = 4;  // PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT
+constexpr int kModels = 4;
+constexpr int kMaxStepsPerLaunch = 8;  // temporal-blocking depth of plbm_run (FP64: 4)
+int max_steps_per_pass(int storage);  // PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT
 constexpr int kReduceBlocks = 1024;
 
 }  // namespace plbm

@@ -319,6 +319,10 @@ int launch_step_cfg(plbm_lattice *h, int model, const void *src, void *dst, Laun
     });
 }
 
+int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst);
+int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew);
+int default_steps_per_launch(const plbm_lattice *h);
+
 int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
@@ -406,6 +410,32 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
             }
         }
         if (status != PLBM_OK) break;
+    }
+    // steps per pass of plbm_run: the single-step kernel at the shape just chosen against
+    // passes of 2..4 steps, compared per lattice update
+    best.ns = 1;
+    for (int ns = 2; ns <= max_steps_per_pass(h->storage) && status == PLBM_OK; ++ns) {
+        double tmin = 1e300;
+        for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
+            status = launch_multi(h, model, ns, h->f[src], scratch);  // warm
+            if (status != PLBM_OK) break;
+            (void)hipEventRecord(e0, h->stream);
+            for (int r = 0; r < reps && status == PLBM_OK; ++r)
+                status = launch_multi(h, model, ns, h->f[src], scratch);
+            (void)hipEventRecord(e1, h->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                set_error("autotune: event synchronisation failed");
+                status = PLBM_ERR_HIP;
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tmin = std::min(tmin, double(ms) / reps / ns);
+        }
+        if (status == PLBM_OK && tmin < tbest) {
+            tbest = tmin;
+            best.ns = ns;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -629,15 +659,14 @@ int plbm_step(plbm_lattice *h, int model, int src, int dst) {
 int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) {
     PLBM_REQUIRE(h && iold && inew, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
+                 "unknown collision model " + std::to_string(model));
     PLBM_TRY(check_buf(h, *iold));
     PLBM_TRY(check_buf(h, *inew));
     PLBM_REQUIRE(*iold != *inew, PLBM_ERR_ARG, "iold == inew");
     PLBM_TRY(require_params(h));
-    for (int s = 0; s < nsteps; ++s) {
-        PLBM_TRY(launch_step(h, model, h->f[*iold], h->f[*inew]));
-        std::swap(*iold, *inew);
-    }
-    return PLBM_OK;
+    const int ns = h->cfg[model].ns > 0 ? h->cfg[model].ns : default_steps_per_launch(h);
+    return run_steps(h, model, ns, nsteps, iold, inew);
 }
 
 int plbm_stream(plbm_lattice *h, int src, int dst) {
@@ -751,10 +780,27 @@ int plbm_sync(plbm_lattice *h) {
 int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
-    LaunchCfg c{block, xcd_remap};
+    LaunchCfg c{block, xcd_remap, h->cfg[model].ns};
     PLBM_REQUIRE(valid_cfg(c), PLBM_ERR_ARG,
                  "block must be a multiple of 64 in [64,1024] and xcd_remap 0 or 1");
     h->cfg[model] = c;
+    return PLBM_OK;
+}
+
+int plbm_set_steps_per_launch(plbm_lattice *h, int model, int ns) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
+    PLBM_REQUIRE(ns >= 0 && ns <= max_steps_per_pass(h->storage), PLBM_ERR_ARG,
+                 "steps per launch must be 0.." + std::to_string(max_steps_per_pass(h->storage)) +
+                     " for this storage");
+    h->cfg[model].ns = ns;
+    return PLBM_OK;
+}
+
+int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *ns) {
+    PLBM_REQUIRE(h && ns, PLBM_ERR_ARG, "null argument");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
+    *ns = h->cfg[model].ns > 0 ? h->cfg[model].ns : default_steps_per_launch(h);
     return PLBM_OK;
 }
 

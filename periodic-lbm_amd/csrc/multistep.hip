@@ -1,0 +1,252 @@
+// multistep.hip — temporal blocking: NS fused stream + collide steps per launch.
+//
+// The single-step kernel reads and writes every population once per step (18 s bytes per
+// lattice update), so it is bound by HBM at one step per pass.  Here one pass advances the
+// lattice NS steps (NS = 2..4) and reads / writes every population once per pass:
+//
+//  * A wavefront owns 64 consecutive rows y and walks a strip of XS columns along x.  Of
+//    its rows, VR produce output; NS rows on each side are halo (VR + 2 NS <= 64).  VR is
+//    a multiple of 8 so that each wave's stores start and end on 64-byte boundaries.
+//  * Level k (k = 0 .. NS-1) keeps the pull-ready values of f^{n+k} for the last three
+//    columns in registers.  Loading column c gives level 0 at c; level k+1 at column
+//    c - k - 1 follows from level k at columns c - k, c - k - 1, c - k - 2 (the pull
+//    f_q(x, y) <- f_q(x - cx_q, y - cy_q) takes x-neighbours from the window and
+//    y-neighbours from the adjacent lanes), and level NS is stored.
+//  * The next column's loads are issued one column ahead (the pass is latency-bound
+//    without it: each wavefront walks its strip sequentially).
+//  * Each level applies exactly the single-step kernel's per-cell arithmetic
+//    (collide_cell), and intermediate levels are rounded to the storage type, so the
+//    result equals NS launches of the single-step kernel bit for bit.
+//
+// Algorithmic traffic per pass: 18 s bytes per cell (one read, one write of each
+// population) for NS lattice updates; the halo re-reads (64/VR in y, (XS+2 NS)/XS in x)
+// are the overhead.  Work per output cell and pass: NS collisions plus the halo ones.
+#include <algorithm>
+#include <string>
+#include <utility>
+
+#include "common.hpp"
+#include "d2q9.hpp"
+#include "stepper.hpp"
+
+namespace plbm {
+
+int launch_step(plbm_lattice *h, int model, const void *src, void *dst);
+
+constexpr int kMultiXS = 64;  // columns per strip
+
+// Rows with output per wave: 56 for 8-byte storage (448-byte aligned runs), 48 for 4- and
+// 2-byte storage (192 / 96 bytes, 64- / 32-byte aligned).  The halo of a pass of NS steps
+// is NS rows on each side, so a pass can hold (64 - VR) / 2 steps: 4 for FP64 storage,
+// 8 for FP32 and FP16 storage (whose registers are half as wide).
+template <int ST>
+constexpr int multi_vr() {
+    return sizeof(typename Traits<ST>::T) == 8 ? 56 : 48;
+}
+template <int ST>
+constexpr int max_steps_per_pass() {
+    return (64 - multi_vr<ST>()) / 2;
+}
+int max_steps_per_pass(int storage) { return storage_bytes(storage) == 8 ? 4 : 8; }
+
+// value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1
+template <class R>
+__device__ __forceinline__ R pull_y(int q, R v) {
+    if (cy(q) > 0) return __shfl_up(v, 1, 64);
+    if (cy(q) < 0) return __shfl_down(v, 1, 64);
+    return v;
+}
+
+template <int MODEL, int ST, int NS>
+__global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int nwaves) {
+    using T = typename Traits<ST>::T;
+    using R = typename Traits<ST>::R;
+    constexpr int VR = multi_vr<ST>();
+    static_assert(VR + 2 * NS <= 64, "halo rows do not fit in a wavefront");
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= nwaves) return;  // the whole wavefront
+    const int nx = a.nx, ny = a.ny;
+    const int wy = gw % waves_y, xsi = gw / waves_y;
+    const int r = VR * wy - NS + lane;  // unwrapped row of this lane
+    int y = r % ny;
+    if (y < 0) y += ny;
+    const bool out = lane >= NS && lane < VR + NS && r < ny;
+    const int x0 = xsi * kMultiXS;
+    const int ncols = min(kMultiXS, nx - x0);
+    const size_t L = (size_t)a.ldd, cs = a.cs;
+    const T *__restrict__ s = static_cast<const T *>(a.src);
+    T *__restrict__ d = static_cast<T *>(a.dst);
+    R W[NS][3][9];  // [level][newest column, -1, -2][q]
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) W[k][j][q] = R(0);
+    auto column = [&](int c) {
+        c %= nx;
+        if (c < 0) c += nx;
+        return (size_t)c * cs;
+    };
+    // the next column's loads are issued one iteration ahead, so that their latency
+    // overlaps the collisions of the current column
+    T nxt[9];
+    {
+        const size_t cc = column(x0 - NS);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) nxt[q] = s[cc + q * L + y];
+    }
+    const int iters = ncols + 2 * NS;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            W[0][2][q] = W[0][1][q];
+            W[0][1][q] = W[0][0][q];
+            W[0][0][q] = pull_y(q, load_pop<ST>(&nxt[q], a.inv_scale));
+        }
+        if (i + 1 < iters) {
+            const size_t cn = column(x0 - NS + i + 1);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) nxt[q] = s[cn + q * L + y];
+        }
+#pragma unroll
+        for (int k = 1; k <= NS; ++k) {
+            if (i < 2 * k) break;
+            R f[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                f[q] = cx(q) > 0 ? W[k - 1][2][q] : (cx(q) == 0 ? W[k - 1][1][q] : W[k - 1][0][q]);
+            collide_cell<MODEL, ST>(f, a);
+            if (k < NS) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    R v = f[q];
+                    if constexpr (ST == PLBM_F16S) v = R(T(v * a.scale)) * a.inv_scale;
+                    W[k][2][q] = W[k][1][q];
+                    W[k][1][q] = W[k][0][q];
+                    W[k][0][q] = pull_y(q, v);
+                }
+            } else if (out) {
+                const size_t co = (size_t)(x0 - 2 * NS + i) * cs;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) store_pop<ST>(d + co + q * L + y, f[q], a.scale);
+            }
+        }
+    }
+}
+
+int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst) {
+    StepArgs a{};
+    a.src = src;
+    a.dst = dst;
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.ldd = h->ldd;
+    a.cs = (size_t)9 * h->ldd;
+    a.omega = h->omega;
+    a.lambda_odd = h->lambda_odd;
+    a.omega_f = float(h->omega);
+    a.lambda_odd_f = float(h->lambda_odd);
+    a.scale = float(h->scale);
+    a.inv_scale = float(1.0 / h->scale);
+    return dispatch_model(model, [&](auto M) {
+        return dispatch_storage(h->storage, [&](auto S) {
+            constexpr int MM = decltype(M)::value, SS = decltype(S)::value;
+            const int vr = multi_vr<SS>();
+            const int waves_y = (h->ny + vr - 1) / vr;
+            const int nw = waves_y * ((h->nx + kMultiXS - 1) / kMultiXS);
+            const dim3 grid((unsigned)((nw + 3) / 4));
+            auto go = [&](auto K) {
+                constexpr int NS = decltype(K)::value;
+                if constexpr (NS <= max_steps_per_pass<SS>())
+                    hipLaunchKernelGGL((k_multi<MM, SS, NS>), grid, dim3(256), 0, h->stream, a,
+                                       waves_y, nw);
+                return NS <= max_steps_per_pass<SS>();
+            };
+            bool ok = false;
+            switch (ns) {
+            case 2: ok = go(std::integral_constant<int, 2>{}); break;
+            case 3: ok = go(std::integral_constant<int, 3>{}); break;
+            case 4: ok = go(std::integral_constant<int, 4>{}); break;
+            case 5: ok = go(std::integral_constant<int, 5>{}); break;
+            case 6: ok = go(std::integral_constant<int, 6>{}); break;
+            case 7: ok = go(std::integral_constant<int, 7>{}); break;
+            case 8: ok = go(std::integral_constant<int, 8>{}); break;
+            default: break;
+            }
+            if (!ok) {
+                set_error("steps per pass " + std::to_string(ns) + " out of range for this storage");
+                return PLBM_ERR_ARG;
+            }
+            PLBM_HIP(hipGetLastError());
+            return PLBM_OK;
+        });
+    });
+}
+
+// nsteps steps of `model` with the reference's bookkeeping (each step reads *iold, writes
+// *inew, swaps them).  With ns > 1 all but the last step run in passes of up to ns steps;
+// the last step is a single-step launch, so that the other buffer ends up holding the
+// state one step before the latest (what update_macros reads, as in the reference).
+// A pass moves the latest state to the other buffer whatever its step count, so when the
+// parity of passes differs from the parity of steps the two slots exchange their device
+// storage at the end.
+int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew) {
+    if (ns <= 1 || nsteps < 2) {
+        for (int s = 0; s < nsteps; ++s) {
+            PLBM_TRY(launch_step(h, model, h->f[*iold], h->f[*inew]));
+            std::swap(*iold, *inew);
+        }
+        return PLBM_OK;
+    }
+    const int a0 = *iold, b0 = *inew;
+    void *latest = h->f[a0], *other = h->f[b0];
+    const int body = nsteps - 1;
+    for (int done = 0; done < body;) {
+        const int k = std::min(ns, body - done);
+        if (k == 1) PLBM_TRY(launch_step(h, model, latest, other));
+        else PLBM_TRY(launch_multi(h, model, k, latest, other));
+        std::swap(latest, other);
+        done += k;
+    }
+    PLBM_TRY(launch_step(h, model, latest, other));
+    std::swap(latest, other);
+    // reference parity: after nsteps swaps the latest state is in slot b0 when nsteps is odd
+    const int want = (nsteps % 2) ? b0 : a0;
+    if (h->f[want] != latest) std::swap(h->f[a0], h->f[b0]);
+    *iold = want;
+    *inew = want == a0 ? b0 : a0;
+    return PLBM_OK;
+}
+
+}  // namespace plbm
+
+namespace plbm {
+
+// Steps per pass when the caller has not chosen: 4 where the grid gives at least 2048
+// wavefronts (8 per CU), else single steps (a small grid cannot fill the chip with strips).
+int default_steps_per_launch(const plbm_lattice *h) {
+    const int vr = storage_bytes(h->storage) == 8 ? 56 : 48;
+    const long waves = (long)((h->ny + vr - 1) / vr) * ((h->nx + kMultiXS - 1) / kMultiXS);
+    return waves >= 2048 ? 4 : 1;
+}
+
+}  // namespace plbm
+
+using namespace plbm;
+
+extern "C" int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
+                 "unknown collision model " + std::to_string(model));
+    PLBM_REQUIRE(steps >= 1 && steps <= max_steps_per_pass(h->storage), PLBM_ERR_ARG,
+                 "steps per pass must be 1.." + std::to_string(max_steps_per_pass(h->storage)) +
+                     " for this storage");
+    PLBM_REQUIRE(src >= 0 && src < h->nf && dst >= 0 && dst < h->nf && src != dst, PLBM_ERR_ARG,
+                 "need distinct valid src and dst buffers");
+    PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE,
+                 "collision parameters not set: call plbm_set_params first");
+    if (steps == 1) return launch_step(h, model, h->f[src], h->f[dst]);
+    return launch_multi(h, model, steps, h->f[src], h->f[dst]);
+}

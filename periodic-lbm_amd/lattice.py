@@ -93,8 +93,25 @@ class Lattice:
                                     ctypes.byref(r)))
         return b.value, r.value
 
+    def set_steps_per_launch(self, model: str | int, steps: int):
+        """Temporal blocking depth of run(): 1 single steps, 2..4 (FP64 storage) or 2..8
+        (FP32 / FP16 storage) fused steps per pass, 0 automatic.  FP64 results are
+        bit-identical for every value."""
+        check(lib().plbm_set_steps_per_launch(self.h, _abi.MODEL_IDS.get(model, model), steps))
+
+    def run_pass(self, model: str | int, steps: int, src: int, dst: int):
+        """One pass of `steps` fused steps src -> dst (no intermediate states stored)."""
+        check(lib().plbm_pass(self.h, _abi.MODEL_IDS.get(model, model), steps, src, dst))
+
+    def get_steps_per_launch(self, model: str | int) -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_get_steps_per_launch(self.h, _abi.MODEL_IDS.get(model, model),
+                                              ctypes.byref(v)))
+        return v.value
+
     def autotune(self, model: str | int, buf: int = 0, reps: int = 10) -> float:
-        """Time every launch shape of the fused step; keep the fastest.  Returns ms/step."""
+        """Time every launch shape of the fused step and the temporal-blocking depths; keep
+        the fastest.  Returns ms per lattice-update step."""
         ms = ctypes.c_double()
         check(lib().plbm_autotune(self.h, _abi.MODEL_IDS.get(model, model), buf, reps,
                                   ctypes.byref(ms)))

@@ -1,0 +1,130 @@
+"""Temporal blocking of plbm_run (passes of 2..4 fused steps per launch, intermediate
+states in registers): in FP64 storage bit-identical to the single-step kernel (and the
+oracle) for every model, pass depth and grid shape (ragged strips and wavefronts, grids
+smaller than one wavefront), with the reference's slot contents after the run (latest
+state in iold, the state one step earlier in inew, which update_macros reads).  The
+shifted storages' collisions allow FMA contraction, which the compiler places per
+kernel, so there the two kernels agree to roundoff."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+MODELS = ["bgk", "trt", "rr", "bgk_split"]
+
+
+def fields(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    return (np.asfortranarray(1 + 0.01 * rng.standard_normal((ny, nx))),
+            np.asfortranarray(0.05 * rng.standard_normal((ny, nx))),
+            np.asfortranarray(0.05 * rng.standard_normal((ny, nx))))
+
+
+def run(P, storage, model, ns, nx, ny, nsteps, seed=1):
+    rho, ux, uy = fields(nx, ny, seed)
+    lat = P.Lattice(nx, ny, 2, storage)
+    lat.set_params(1.0 / (3 * 0.004 + 0.5), 0.25)
+    lat.set_steps_per_launch(model, ns)
+    assert lat.get_steps_per_launch(model) == ns
+    lat.equilibrium_init(0, rho, ux, uy)
+    iold, inew = lat.run(model, nsteps, 0, 1)
+    return lat, iold, inew
+
+
+# 200 x 130: ragged last strip (200 = 3*64 + 8) and ragged last wavefront rows;
+# 5 x 3 and 1 x 7: the whole grid inside one wavefront, rows wrapping several times
+@pytest.mark.parametrize("nx,ny", [(200, 130), (64, 64), (5, 3), (1, 7), (70, 1)])
+@pytest.mark.parametrize("model", MODELS)
+def test_passes_equal_single_steps_fp64(P, gpu, oracle, model, nx, ny):
+    print(f"case multistep f64 {model} {nx}x{ny}", flush=True)
+    nsteps = 11
+    ref, ri, rn = run(P, "f64", model, 1, nx, ny, nsteps)
+    want_new, want_prev = ref.download_f(ri), ref.download_f(rn)
+    assert (ri, rn) == (1, 0)  # 11 steps from (0, 1): the reference's parity
+    for ns in (2, 3, 4):
+        lat, i, n = run(P, "f64", model, ns, nx, ny, nsteps)
+        assert (i, n) == (ri, rn)
+        np.testing.assert_array_equal(lat.download_f(i), want_new)
+        np.testing.assert_array_equal(lat.download_f(n), want_prev)
+        np.testing.assert_array_equal(np.stack(lat.macros(n)), np.stack(ref.macros(rn)))
+    # and the single-step kernel is the oracle
+    rho, ux, uy = fields(nx, ny, 1)
+    fa, fb = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, fa)
+    latest, prev = oracle.run(model, nx, ny, fa, fb, nsteps, 1.0 / (3 * 0.004 + 0.5), 0.25)
+    np.testing.assert_array_equal(want_new[:ny], latest[:ny])
+
+
+# velocity tolerance after 10 steps: FP64 roundoff, FP32 roundoff, one FP16 storage
+# rounding (scale 2^10) of an intermediate state
+SHIFTED_TOL = {"f64s": 1e-14, "f32s": 1e-6, "f16s": 2e-4}
+
+
+@pytest.mark.parametrize("storage", ["f64s", "f32s", "f16s"])
+@pytest.mark.parametrize("model", ["bgk", "trt", "rr"])
+def test_passes_match_single_steps_shifted(P, gpu, storage, model):
+    nx, ny, nsteps = 150, 100, 10
+    ref, ri, rn = run(P, storage, model, 1, nx, ny, nsteps)
+    r0, u0, v0 = ref.macros(ri)
+    for ns in ((2, 3, 4) if storage == "f64s" else (2, 3, 4, 5, 6, 7, 8)):
+        lat, i, n = run(P, storage, model, ns, nx, ny, nsteps)
+        assert (i, n) == (ri, rn)
+        r, u, v = lat.macros(i)
+        tol = SHIFTED_TOL[storage]
+        np.testing.assert_allclose(u, u0, rtol=0, atol=tol)
+        np.testing.assert_allclose(v, v0, rtol=0, atol=tol)
+        np.testing.assert_allclose(r, r0, rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 3, 4, 5, 6, 9])
+def test_step_counts_and_slots(P, gpu, nsteps):
+    """Every step count (single passes, partial last pass, exact multiples) keeps the
+    reference's slot contents; the slots' device storage may be exchanged."""
+    nx, ny = 96, 80
+    ref, ri, rn = run(P, "f64", "trt", 1, nx, ny, nsteps)
+    lat, i, n = run(P, "f64", "trt", 4, nx, ny, nsteps)
+    assert (i, n) == (ri, rn) == ((1, 0) if nsteps % 2 else (0, 1))
+    np.testing.assert_array_equal(lat.download_f(i), ref.download_f(ri))
+    np.testing.assert_array_equal(lat.download_f(n), ref.download_f(rn))
+    # a second run continues from the returned ids
+    i2, n2 = lat.run("trt", 5, i, n)
+    r2, q2 = ref.run("trt", 5, ri, rn)
+    np.testing.assert_array_equal(lat.download_f(i2), ref.download_f(r2))
+
+
+def test_full_size_passes_bit_exact_4096(P, gpu, oracle):
+    """4096^2 TRT, 9 steps in passes of 4 (the automatic choice at this size) against
+    the oracle."""
+    n, nsteps = 4096, 9
+    print(f"case multistep 4096 trt {nsteps}", flush=True)
+    rho, ux, uy = fields(n, n, 5)
+    lat = P.Lattice(n, n, 2, "f64")
+    omega = 1.0 / (3 * 0.01 + 0.5)
+    lat.set_params(omega, 0.25)
+    assert lat.get_steps_per_launch("trt") == 4
+    lat.equilibrium_init(0, rho, ux, uy)
+    i, n_ = lat.run("trt", nsteps, 0, 1)
+    fa, fb = po.new_f(n, n), po.new_f(n, n)
+    oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, fa)
+    latest, prev = oracle.run("trt", n, n, fa, fb, nsteps, omega, 0.25)
+    np.testing.assert_array_equal(lat.download_f(i)[:n], latest[:n])
+    np.testing.assert_array_equal(lat.download_f(n_)[:n], prev[:n])
+
+
+def test_autotune_picks_a_depth(P, gpu):
+    n = 2048
+    lat = P.Lattice(n, n, 2, "f64")
+    lat.set_params(1.9, 0.25)
+    lat.equilibrium_init(0, *fields(n, n, 2))
+    ms = lat.autotune("trt", 0, reps=3)
+    assert ms > 0 and lat.get_steps_per_launch("trt") in (1, 2, 3, 4)
+    with pytest.raises(P.PlbmError):
+        lat.set_steps_per_launch("trt", 5)  # FP64 passes hold at most 4 steps
+    lat32 = P.Lattice(64, 64, 2, "f32s")
+    lat32.set_steps_per_launch("trt", 8)
+    with pytest.raises(P.PlbmError):
+        lat32.set_steps_per_launch("trt", 9)
