@@ -222,7 +222,9 @@ def schemes_leg(P, torch, args, stream, fields):
 def batch_leg(P, torch, args):
     """Uneven-size sweep: 256 periodic cases (n from {32..512}, nu log-uniform in
     [1e-3, 5e-2], half Taylor-Green, half shear layers), BGK, FP32-shifted storage, one
-    autotuned launch plan, hipGraph replay; the same cases in FP64 give the error."""
+    autotuned launch plan (tile shape or steps per pass, resident threshold), hipGraph
+    replay; the same cases in FP64 give the error.  Bytes per lattice update: 72 / steps
+    per pass (one read and one write of the FP32 populations per pass)."""
     import numpy as np
     cases = P.sweep_cases(256, seed=4)
     out = {"ncases": len(cases), "steps": args.batch_steps, "storage": "f32s", "model": "bgk"}
@@ -231,6 +233,7 @@ def batch_leg(P, torch, args):
     out["cells"] = info["cells"]
     out["tune_ms_per_step"] = b.autotune(10)
     out["launch"] = dict(zip(("ty", "tx", "resident_cells", "resident_block"), b.get_launch()))
+    out["launch"]["steps_per_pass"] = b.get_steps_per_pass()
     b.run(20)
     b.sync()
     t0 = time.perf_counter()
@@ -239,7 +242,8 @@ def batch_leg(P, torch, args):
     dt = time.perf_counter() - t0
     out["seconds"] = dt
     out["mlups"] = info["cells"] * args.batch_steps / dt / 1e6
-    out["gbs_at_72B_per_lu"] = out["mlups"] * 1e6 * 72 / 1e9
+    out["bytes_per_lu"] = 72 / out["launch"]["steps_per_pass"]
+    out["algorithmic_gbs"] = out["mlups"] * 1e6 * out["bytes_per_lu"] / 1e9
     red32 = b.reduce()
     b.close()
     b64 = P.Batch.from_cases(cases, "f64", "bgk")

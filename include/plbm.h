@@ -275,8 +275,14 @@ int plbm_batch_sync(plbm_batch *b);
 int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int resident_block);
 int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
                           int *resident_block);
-/* Autotuner: time `steps` (even) steps of every plan candidate on a scratch copy of the
- * arena and keep the fastest; *best_ms (may be NULL) = its time per step. */
+/* Steps per launch of the tiled (non-resident) cases: 1 = one step per launch (tile
+ * table); 2..4 (FP64 / F64S) or 2..8 (F32S / F16S) = temporally blocked passes, as
+ * plbm_set_steps_per_launch for one lattice (bit-identical in F64 storage).  Default 1. */
+int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps);
+int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps);
+/* Autotuner: time `steps` (even) steps of every plan candidate (tile shapes, steps per
+ * pass, resident threshold) on a scratch copy of the arena and keep the fastest;
+ * *best_ms (may be NULL) = its time per step. */
 int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms);
 
 /* ---------------------------------------------------------------- adjoint */

@@ -140,6 +140,16 @@ class Batch:
         check(lib().plbm_batch_get_launch(self.h, *(ctypes.byref(x) for x in v)))
         return tuple(x.value for x in v)
 
+    def set_steps_per_pass(self, steps: int):
+        """Steps per launch of the tiled cases (1: one step per launch; 2..4 FP64, 2..8
+        FP32 / FP16 storage: temporally blocked passes)."""
+        check(lib().plbm_batch_set_steps_per_pass(self.h, steps))
+
+    def get_steps_per_pass(self) -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_batch_get_steps_per_pass(self.h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, steps=10) -> float:
         ms = ctypes.c_double()
         check(lib().plbm_batch_autotune(self.h, steps, ctypes.byref(ms)))

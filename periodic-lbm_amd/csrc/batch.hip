@@ -14,8 +14,14 @@
 //    the XCD's L2 for the whole run instead of streaming through HBM every step.  It is
 //    captured into the same graph on a forked stream, so it overlaps the big-case steps.
 //
-// Tile shape (ty, tx) and the resident threshold are picked by the project's autotuner
-// (plbm_batch_autotune), which times candidates on a scratch copy of the arena.
+//  * k_batch_multi (steps per pass ns > 1): the tiled cases advance ns steps per launch
+//    with the temporally blocked pass of the single lattice (multistep.hpp), one
+//    wavefront per 56 x 64 (FP64) or 48 x 64 (FP32 / FP16) block of a case, listed in a
+//    wave table in place of the tile table.
+//
+// Tile shape (ty, tx), steps per pass and the resident threshold are picked by the
+// project's autotuner (plbm_batch_autotune), which times candidates on a scratch copy of
+// the arena.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -25,6 +31,7 @@
 
 #include "common.hpp"
 #include "d2q9.hpp"
+#include "multistep.hpp"
 #include "stepper.hpp"
 
 namespace plbm {
@@ -45,6 +52,7 @@ struct BatchCfg {
     int ty = 128, tx = 1;      // tile = ty x tx cells, one thread each
     int resident_cells = 0;    // cases with nx*ny <= this run in k_batch_resident
     int resident_block = 256;
+    int ns = 1;                // steps per pass of the tiled cases (> 1: k_batch_multi)
 };
 
 template <int ST>
@@ -110,6 +118,24 @@ __global__ void __launch_bounds__(1024)
     T *d = parity ? base : base + c.elems;
     const StepArgs a = case_args<ST>(c, scale, inv_scale);
     batch_cell<MODEL, ST, true>(s, d, a, x, y);
+}
+
+// One pass of NS steps of every tiled case; waves[i] = {case, strip, row block}, four
+// wavefronts per workgroup.  parity 0: A -> B, 1: B -> A.
+template <int MODEL, int ST, int NS>
+__global__ void __launch_bounds__(256)
+    k_batch_multi(const BatchCase *__restrict__ cases, const BatchTile *__restrict__ waves,
+                  int nwaves, void *arena, int parity, float scale, float inv_scale) {
+    using T = typename Traits<ST>::T;
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= nwaves) return;  // the whole wavefront
+    const BatchTile w = waves[gw];
+    const BatchCase c = cases[w.c];
+    T *base = static_cast<T *>(arena) + c.off;
+    StepArgs a = case_args<ST>(c, scale, inv_scale);
+    a.src = parity ? base + c.elems : base;
+    a.dst = parity ? base : base + c.elems;
+    multi_wave<MODEL, ST, NS>(a, w.y0, w.x0);
 }
 
 // nsteps steps of one small case per workgroup; only workgroup barriers between steps.
@@ -279,8 +305,14 @@ static int build_plan(plbm_batch *b, const BatchCfg &cfg, BatchTile **d_tiles, i
             res.push_back(c);
             continue;
         }
-        for (int x0 = 0; x0 < k.nx; x0 += cfg.tx)
-            for (int y0 = 0; y0 < k.ny; y0 += cfg.ty) tiles.push_back({c, x0, y0, 0});
+        if (cfg.ns > 1) {  // wave table: {case, strip, row block}
+            const int vr = storage_bytes(b->storage) == 8 ? 56 : 48;
+            for (int xs = 0; xs * kMultiXS < k.nx; ++xs)
+                for (int wy = 0; wy * vr < k.ny; ++wy) tiles.push_back({c, xs, wy, 0});
+        } else {
+            for (int x0 = 0; x0 < k.nx; x0 += cfg.tx)
+                for (int y0 = 0; y0 < k.ny; y0 += cfg.ty) tiles.push_back({c, x0, y0, 0});
+        }
     }
     *ntiles = (int)tiles.size();
     *nres = (int)res.size();
@@ -314,6 +346,21 @@ static int set_cfg(plbm_batch *b, const BatchCfg &cfg) {
     return PLBM_OK;
 }
 
+// Pass depths (each <= ns) summing to nsteps whose count has the parity of nsteps.
+static std::vector<int> pass_depths(int ns, int nsteps) {
+    std::vector<int> d;
+    for (int left = nsteps; left > 0; left -= d.back()) d.push_back(std::min(ns, left));
+    if ((int)(d.size() % 2) != nsteps % 2) {
+        for (size_t i = 0; i < d.size(); ++i)
+            if (d[i] >= 2) {
+                d[i] -= 1;
+                d.insert(d.begin() + (long)i + 1, 1);
+                break;
+            }
+    }
+    return d;
+}
+
 // Enqueue `nsteps` steps starting at `parity` on b->stream (resident cases on b->side).
 static int enqueue_steps(plbm_batch *b, void *arena, const BatchTile *tiles, int ntiles,
                          const int *res, int nres, const BatchCfg &cfg, int parity,
@@ -329,12 +376,38 @@ static int enqueue_steps(plbm_batch *b, void *arena, const BatchTile *tiles, int
                                    0, b->side, b->d_cases, res, arena, parity, nsteps, sc, isc);
                 PLBM_HIP(hipGetLastError());
             }
-            if (ntiles > 0) {
+            if (ntiles > 0 && cfg.ns <= 1) {
                 for (int s = 0; s < nsteps; ++s) {
                     hipLaunchKernelGGL((k_batch_step<MM, SS>), dim3(ntiles), dim3(cfg.ty * cfg.tx), 0,
                                        b->stream, b->d_cases, tiles, arena, (parity + s) & 1,
                                        cfg.ty, sc, isc);
                     PLBM_HIP(hipGetLastError());
+                }
+            } else if (ntiles > 0) {
+                // passes of up to ns steps; every pass moves the state to the other buffer,
+                // so the number of passes must have the parity of nsteps (as the resident
+                // cases, which flip once per step): split a pass in two when it has not
+                for (int d : pass_depths(cfg.ns, nsteps)) {
+                    const dim3 grid((unsigned)((ntiles + 3) / 4));
+                    auto go = [&](auto K) {
+                        constexpr int NS = decltype(K)::value;
+                        if constexpr (NS <= max_steps_per_pass<SS>())
+                            hipLaunchKernelGGL((k_batch_multi<MM, SS, NS>), grid, dim3(256), 0,
+                                               b->stream, b->d_cases, tiles, ntiles, arena,
+                                               parity, sc, isc);
+                    };
+                    switch (d) {
+                    case 1: go(std::integral_constant<int, 1>{}); break;
+                    case 2: go(std::integral_constant<int, 2>{}); break;
+                    case 3: go(std::integral_constant<int, 3>{}); break;
+                    case 4: go(std::integral_constant<int, 4>{}); break;
+                    case 5: go(std::integral_constant<int, 5>{}); break;
+                    case 6: go(std::integral_constant<int, 6>{}); break;
+                    case 7: go(std::integral_constant<int, 7>{}); break;
+                    default: go(std::integral_constant<int, 8>{}); break;
+                    }
+                    PLBM_HIP(hipGetLastError());
+                    parity ^= 1;
                 }
             }
             if (nres > 0) {
@@ -617,12 +690,29 @@ int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int
                      resident_block % 64 == 0,
                  PLBM_ERR_ARG, "resident block must be a multiple of 64 in [64,1024]");
     PLBM_HIP(hipStreamSynchronize(b->stream));
-    BatchCfg c;
+    BatchCfg c = b->cfg;
     c.ty = ty;
     c.tx = tx;
     c.resident_cells = resident_cells;
     c.resident_block = resident_block;
     return set_cfg(b, c);
+}
+
+int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps) {
+    PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
+    PLBM_REQUIRE(steps >= 1 && steps <= max_steps_per_pass(b->storage), PLBM_ERR_ARG,
+                 "steps per pass must be 1.." + std::to_string(max_steps_per_pass(b->storage)) +
+                     " for this storage");
+    PLBM_HIP(hipStreamSynchronize(b->stream));
+    BatchCfg c = b->cfg;
+    c.ns = steps;
+    return set_cfg(b, c);
+}
+
+int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps) {
+    PLBM_REQUIRE(b && steps, PLBM_ERR_ARG, "null argument");
+    *steps = b->cfg.ns;
+    return PLBM_OK;
 }
 
 int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
@@ -699,6 +789,17 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
                 BatchCfg c;
                 c.ty = ty;
                 c.tx = tx;
+                c.resident_cells = res;
+                c.resident_block = rb;
+                cands.push_back(c);
+            }
+    const int max_ns = max_steps_per_pass(b->storage);
+    for (int ns = 2; ns <= max_ns; ns += (ns < 4 ? 1 : 2))
+        for (int res : {0, 4096, 16384, 36864})
+            for (int rb : {256, 1024}) {
+                if (res == 0 && rb != 256) continue;
+                BatchCfg c;
+                c.ns = ns;
                 c.resident_cells = res;
                 c.resident_block = rb;
                 cands.push_back(c);
