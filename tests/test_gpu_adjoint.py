@@ -79,9 +79,9 @@ def test_autograd_function(P, gpu):
     assert float(tau.grad) == pytest.approx(-gom / TAU ** 2, rel=1e-15)
     # one gradient-descent step on the initial field lowers the misfit
     with torch.no_grad():
-        lr = 0.5 * float(loss) / float(torch.sum(ux.grad ** 2) + torch.sum(uy.grad ** 2))
+        lr = 0.5 * float(loss.detach()) / float(torch.sum(ux.grad ** 2) + torch.sum(uy.grad ** 2))
         l_new = P.velocity_misfit(ux - lr * ux.grad, uy - lr * uy.grad, tau, uxt, uyt, eng)
-    assert float(l_new) < float(loss)
+    assert float(l_new) < float(loss.detach())
 
 
 def test_checkpoint_interval_does_not_change_gradients(P, gpu):
