@@ -37,13 +37,15 @@ module fvm_bardow
    public :: cx, cy, csqr
 
    ! engine extensions (prefixed so they cannot clash with program names)
-   public :: plbm_sync_host, plbm_sync_device, plbm_fused_model
+   public :: plbm_sync_host, plbm_sync_device, plbm_fused_model, plbm_perform_steps
+   public :: plbm_set_steps_per_pass
    public :: plbm_lbm_stream, plbm_collide_bgk, plbm_collide_trt, plbm_collide_rr
    public :: plbm_collide_bgk_split
    public :: plbm_reductions
    public :: plbm_grid_dugks_collide, plbm_grid_dugks_stream, plbm_grid_perform_dugks_step
    public :: plbm_tau_over_dt
    public :: PLBM_F64, PLBM_F64S, PLBM_F32S, PLBM_F16S, PLBM_BGK_SPLIT
+   public :: PLBM_BGK, PLBM_TRT, PLBM_RR
 
    integer, parameter :: WHERE_HOST = 0, WHERE_DEVICE = 1
 
@@ -432,6 +434,43 @@ contains
       itmp = grid%iold
       grid%iold = grid%inew
       grid%inew = itmp
+   end subroutine
+
+   !> Engine extension: steps per pass of plbm_perform_steps for one collision model
+   !> (PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT); 0 = chosen from the grid size.
+   subroutine plbm_set_steps_per_pass(grid, model, steps)
+      type(lattice_grid), intent(inout) :: grid
+      integer, intent(in) :: model, steps
+      call plbm_check(plbm_set_steps_per_launch(grid%dev, int(model,c_int), int(steps,c_int)), &
+         "plbm_set_steps_per_pass")
+   end subroutine
+
+   !> Engine extension: the effect of `nsteps` calls of perform_step in one call.  With
+   !> lbm_stream and an engine collision in the slots the steps run in temporally blocked
+   !> passes of several steps per sweep over device memory (plbm_run; bit-identical to
+   !> single steps in FP64 storage, the slots end as after nsteps perform_step calls);
+   !> any other pair is stepped one perform_step at a time.
+   subroutine plbm_perform_steps(grid, nsteps)
+      type(lattice_grid), intent(inout) :: grid
+      integer, intent(in) :: nsteps
+      integer :: model, s
+      integer(c_int) :: iold, inew
+
+      model = plbm_fused_model(grid)
+      if (model >= 0 .and. plbm_fused_op(grid) == PLBM_STREAM_EXACT) then
+         call require_device(grid)
+         iold = int(grid%iold - 1, c_int)
+         inew = int(grid%inew - 1, c_int)
+         call plbm_check(plbm_run(grid%dev, int(model,c_int), int(nsteps,c_int), iold, inew), &
+            "plbm_perform_steps")
+         call device_written(grid)
+         grid%iold = iold + 1
+         grid%inew = inew + 1
+      else
+         do s = 1, nsteps
+            call perform_step(grid)
+         end do
+      end if
    end subroutine
 
    !> perform_triple_step (src/fvm_bardow.F90:322-340): keeps the pre-collision state.

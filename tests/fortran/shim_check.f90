@@ -1,6 +1,6 @@
 ! Semantics of the engine's Fortran modules beyond the unchanged app: the dispatch of
 ! user procedures in the grid%collision slot (host mirror), perform_triple_step, the
-! reduced-precision storage argument and the FP64 reductions.  Prints one line per check
+! reduced-precision storage argument, the FP64 reductions and plbm_perform_steps.  Prints one line per check
 ! and stops with an error on the first failure.
 module user_ops
    use fvm_bardow
@@ -97,6 +97,11 @@ program shim_check
    err = abs(red(0) - sum(a%f(1:n,:,:,a%iold)))/red(0)
    call check("plbm_sync_device after a host edit", err < 1.0e-13_wp, err)
 
+   ! 6. plbm_perform_steps (passes of several steps) == perform_step called one by one,
+   !    both buffers, on a grid large enough for passes (240 x 200: ragged strips)
+   call check_perform_steps(37)
+   call check_perform_steps(40)
+
    call dealloc_grid(a)
    call dealloc_grid(b)
    call dealloc_grid(c)
@@ -115,6 +120,46 @@ contains
       call tg%eval(0.0_wp, g%rho, g%ux, g%uy)
       g%rho = g%rho/csqr + 1.0_wp
       call set_pdf_to_equilibrium(g)
+   end subroutine
+
+   subroutine check_perform_steps(ns)
+      integer, intent(in) :: ns
+      type(lattice_grid) :: p, q
+      integer :: k, x, y
+      character(len=64) :: what
+      do k = 1, 2
+         if (k == 1) call alloc_grid(p, 240, 200, nf=2, log=.false.)
+         if (k == 2) call alloc_grid(q, 240, 200, nf=2, log=.false.)
+      end do
+      call set_properties(p, 0.01_wp, 1.0_wp, 0.25_wp)
+      call set_properties(q, 0.01_wp, 1.0_wp, 0.25_wp)
+      call plbm_set_steps_per_pass(p, PLBM_TRT, 4)
+      p%streaming => lbm_stream
+      q%streaming => lbm_stream
+      p%collision => collide_trt
+      q%collision => collide_trt
+      do x = 1, 240
+         do y = 1, 200
+            p%rho(y,x) = 1.0_wp + 1.0e-3_wp*sin(0.1_wp*x)*cos(0.2_wp*y)
+            p%ux(y,x) = 0.04_wp*cos(0.05_wp*x + 0.1_wp*y)
+            p%uy(y,x) = -0.03_wp*sin(0.07_wp*x - 0.03_wp*y)
+         end do
+      end do
+      q%mf = p%mf
+      call set_pdf_to_equilibrium(p)
+      call set_pdf_to_equilibrium(q)
+      call plbm_perform_steps(p, ns)
+      do k = 1, ns
+         call perform_step(q)
+      end do
+      call plbm_sync_host(p)
+      call plbm_sync_host(q)
+      err = max(maxval(abs(p%f(1:200,:,:,p%iold) - q%f(1:200,:,:,q%iold))), &
+                maxval(abs(p%f(1:200,:,:,p%inew) - q%f(1:200,:,:,q%inew))))
+      write(what, '(A,I0,A)') "plbm_perform_steps(", ns, ") == perform_step x n"
+      call check(trim(what), err == 0.0_wp .and. p%iold == q%iold, err)
+      call dealloc_grid(p)
+      call dealloc_grid(q)
    end subroutine
 
    subroutine check(what, ok, val)
