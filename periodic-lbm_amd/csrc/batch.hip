@@ -318,14 +318,26 @@ static int build_plan(plbm_batch *b, const BatchCfg &cfg, BatchTile **d_tiles, i
     *nres = (int)res.size();
     *d_tiles = nullptr;
     *d_res = nullptr;
+    auto fail = [&](hipError_t e) {
+        if (*d_tiles) (void)hipFree(*d_tiles);
+        if (*d_res) (void)hipFree(*d_res);
+        *d_tiles = nullptr;
+        *d_res = nullptr;
+        set_error(std::string("batch launch plan: ") + hipGetErrorString(e));
+        return PLBM_ERR_HIP;
+    };
+    hipError_t e;
     if (!tiles.empty()) {
-        PLBM_HIP(hipMalloc(d_tiles, tiles.size() * sizeof(BatchTile)));
-        PLBM_HIP(hipMemcpy(*d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile),
-                           hipMemcpyHostToDevice));
+        if ((e = hipMalloc(d_tiles, tiles.size() * sizeof(BatchTile))) != hipSuccess) return fail(e);
+        if ((e = hipMemcpy(*d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e);
     }
     if (!res.empty()) {
-        PLBM_HIP(hipMalloc(d_res, res.size() * sizeof(int)));
-        PLBM_HIP(hipMemcpy(*d_res, res.data(), res.size() * sizeof(int), hipMemcpyHostToDevice));
+        if ((e = hipMalloc(d_res, res.size() * sizeof(int))) != hipSuccess) return fail(e);
+        if ((e = hipMemcpy(*d_res, res.data(), res.size() * sizeof(int),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e);
     }
     return PLBM_OK;
 }

@@ -464,6 +464,22 @@ def perform_step(grid: LatticeGrid) -> None:
 perform_lbm_step = perform_step
 
 
+def perform_steps(grid: LatticeGrid, nsteps: int) -> None:
+    """Engine extension: the effect of `nsteps` perform_step calls in one call.  With
+    lbm_stream and an engine collision the steps run in temporally blocked passes
+    (Lattice.run; bit-identical to single steps in FP64 storage, same slot contents);
+    any other pair is stepped one perform_step at a time."""
+    model = _ENGINE_COLLISIONS.get(grid.collision)
+    if model is not None and grid.streaming is lbm_stream:
+        _require_device(grid)
+        iold, inew = grid.lattice.run(model, nsteps, grid.iold - 1, grid.inew - 1)
+        _device_written(grid)
+        grid.iold, grid.inew = iold + 1, inew + 1
+    else:
+        for _ in range(nsteps):
+            perform_step(grid)
+
+
 def perform_triple_step(grid: LatticeGrid) -> None:
     """perform_triple_step (src/fvm_bardow.F90:322-340)."""
     _dispatch(grid, grid.streaming, _ENGINE_STREAMING)

@@ -128,3 +128,28 @@ def test_autotune_picks_a_depth(P, gpu):
     lat32.set_steps_per_launch("trt", 8)
     with pytest.raises(P.PlbmError):
         lat32.set_steps_per_launch("trt", 9)
+
+
+@pytest.mark.parametrize("nsteps", [9, 10])
+def test_reference_named_perform_steps(P, gpu, nsteps):
+    """perform_steps(grid, n) == n x perform_step (both slots, ids) through the
+    reference-named API, in passes of 4 on a 300 x 200 grid."""
+    nx, ny = 300, 200
+    rho, ux, uy = fields(nx, ny, 8)
+    grids = []
+    for _ in range(2):
+        g = P.alloc_grid(nx, ny, 2, log=False)
+        g.mf[:, :, 0], g.mf[:, :, 1], g.mf[:, :, 2] = rho, ux, uy
+        P.set_properties(g, 0.01, 1.0, 0.25)
+        g.streaming, g.collision = P.lbm_stream, P.collide_rr
+        P.set_pdf_to_equilibrium(g)
+        grids.append(g)
+    grids[0].lattice.set_steps_per_launch("rr", 4)
+    P.perform_steps(grids[0], nsteps)
+    for _ in range(nsteps):
+        P.perform_step(grids[1])
+    a, b = grids
+    assert (a.iold, a.inew) == (b.iold, b.inew)
+    P.sync_host(a)
+    P.sync_host(b)
+    np.testing.assert_array_equal(a.f[:ny], b.f[:ny])
