@@ -202,6 +202,9 @@ def schemes_leg(P, torch, args, stream, fields):
     out = {"grid": [n, n], "storage": "f64", "steps": steps}
     ms = timed(lambda k: lat.dugks_step(0, 1, k, 1.0, om_full, om_half), steps, 4)
     out["dugks_fused"] = rec(ms, 360)
+    out["dugks_fused"]["steps_per_pass"] = lat.dugks_get_steps_per_pass()
+    out["dugks_fused"]["note"] = ("all steps but the last in temporally blocked passes; "
+                                  "bytes_per_cell is that of one collide + flux step")
 
     def dugks_ref(k):
         for s in range(k):

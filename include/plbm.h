@@ -162,14 +162,20 @@ int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt
  *                       are on entry.  On return every slot holds what the reference's
  *                       buffer of the same index holds after the same steps (the caller
  *                       swaps its iold/inew ids once per step, as the reference does).
- *                       The fused form moves 360 bytes per cell per step; when nsteps is
- *                       odd the two slots exchange their device storage, so addresses from
+ *                       The last step moves 360 bytes per cell; the steps before it run in
+ *                       temporally blocked passes (plbm_dugks_set_steps_per_pass).  The
+ *                       two slots may exchange their device storage, so addresses from
  *                       plbm_buffer_ptr for them must be fetched again.  Asynchronous. */
 int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omega_full,
                        double omega_half);
 int plbm_dugks_stream(plbm_lattice *h, int src, int dst, double dt);
 int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, double dt,
                     double omega_full, double omega_half);
+/* Steps per pass of plbm_dugks_step: all steps but the last run in temporally blocked
+ * passes of up to this many steps over device memory (1 = the collide + flux pair every
+ * step; 2..3 = passes; 0 = chosen from the grid size).  Bit-identical for every value. */
+int plbm_dugks_set_steps_per_pass(plbm_lattice *h, int steps);
+int plbm_dugks_get_steps_per_pass(const plbm_lattice *h, int *steps);
 
 /* ---------------------------------------------------------------- read-back */
 

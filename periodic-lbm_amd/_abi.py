@@ -113,6 +113,8 @@ SIGNATURES = {
     "plbm_dugks_collide": (_i, [_vp, _i, _i, _d, _d]),
     "plbm_dugks_stream": (_i, [_vp, _i, _i, _d]),
     "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),
+    "plbm_dugks_set_steps_per_pass": (_i, [_vp, _i]),
+    "plbm_dugks_get_steps_per_pass": (_i, [_vp, _ip]),
     "plbm_field_barotropic_vortex": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _d, _dp, _dp, _dp]),
 }
 STREAM_OPS = {"exact": 0, "fvm_bardow": 1, "fdm_bardow": 2, "fdm_sofonea": 3}

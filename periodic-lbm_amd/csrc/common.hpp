@@ -153,6 +153,7 @@ struct plbm_lattice {
     double scale = 1.0;            // FP16 storage scale
     plbm::LaunchCfg cfg[plbm::kModels];  // fused-step launch shape per collision model
     bool tuned[plbm::kModels] = {};
+    int dugks_ns = 0;                    // DUGKS steps per pass (0 = chosen from the grid)
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

@@ -174,6 +174,16 @@ class Lattice:
         """dugks_stream: dst -= dt c_q . face flux balance of src."""
         check(lib().plbm_dugks_stream(self.h, src, dst, dt))
 
+    def dugks_set_steps_per_pass(self, steps: int):
+        """Passes of plbm_dugks_step: 1 = collide + flux every step, 2..3 = temporally
+        blocked passes, 0 = automatic (bit-identical for every value)."""
+        check(lib().plbm_dugks_set_steps_per_pass(self.h, steps))
+
+    def dugks_get_steps_per_pass(self) -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_dugks_get_steps_per_pass(self.h, ctypes.byref(v)))
+        return v.value
+
     def dugks_step(self, iold: int, inew: int, nsteps: int, dt: float, omega_full: float,
                    omega_half: float):
         """nsteps x perform_dugks_step on slots (iold, inew); slot contents afterwards

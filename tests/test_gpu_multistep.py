@@ -153,3 +153,34 @@ def test_reference_named_perform_steps(P, gpu, nsteps):
     P.sync_host(a)
     P.sync_host(b)
     np.testing.assert_array_equal(a.f[:ny], b.f[:ny])
+
+
+def test_pass_and_run_errors(P, gpu):
+    lat = P.Lattice(64, 64, 2, "f64")
+    with pytest.raises(P.PlbmError, match="parameters not set"):
+        lat.run_pass("trt", 2, 0, 1)
+    lat.set_params(1.5, 0.25)
+    with pytest.raises(P.PlbmError, match="distinct"):
+        lat.run_pass("trt", 2, 1, 1)
+    with pytest.raises(P.PlbmError, match="steps per pass"):
+        lat.run_pass("trt", 5, 0, 1)
+    with pytest.raises(P.PlbmError, match="unknown collision"):
+        lat.run_pass(7, 2, 0, 1)
+    with pytest.raises(P.PlbmError, match="unknown collision"):
+        lat.run(7, 4, 0, 1)
+    with pytest.raises(P.PlbmError):
+        lat.step_op("fvm_bardow", "trt", 0, 1, float("nan"))
+    with pytest.raises(P.PlbmError):
+        lat.step_op(9, "trt", 0, 1, 1.0)
+    b = P.Batch([(32, 32)], "f64", "bgk")
+    with pytest.raises(P.PlbmError, match="steps per pass"):
+        b.set_steps_per_pass(5)
+    # a valid call after the errors still works and equals single steps
+    lat.equilibrium_init(0, *fields(64, 64, 3))
+    lat.run_pass("trt", 2, 0, 1)
+    ref = P.Lattice(64, 64, 2, "f64")
+    ref.set_params(1.5, 0.25)
+    ref.equilibrium_init(0, *fields(64, 64, 3))
+    ref.set_steps_per_launch("trt", 1)
+    i, _ = ref.run("trt", 2, 0, 1)
+    np.testing.assert_array_equal(lat.download_f(1), ref.download_f(i))
