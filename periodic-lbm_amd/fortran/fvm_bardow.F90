@@ -38,7 +38,7 @@ module fvm_bardow
 
    ! engine extensions (prefixed so they cannot clash with program names)
    public :: plbm_sync_host, plbm_sync_device, plbm_fused_model, plbm_perform_steps
-   public :: plbm_set_steps_per_pass
+   public :: plbm_set_steps_per_pass, plbm_perform_dugks_steps
    public :: plbm_lbm_stream, plbm_collide_bgk, plbm_collide_trt, plbm_collide_rr
    public :: plbm_collide_bgk_split
    public :: plbm_reductions
@@ -629,6 +629,33 @@ contains
       itmp = grid%iold
       grid%iold = grid%inew
       grid%inew = itmp
+   end subroutine
+
+   !> Engine extension: the effect of `nsteps` perform_dugks_step calls in one call; with
+   !> dugks_collide + dugks_stream in the slots all steps but the last run in temporally
+   !> blocked passes (bit-identical, same slot contents as the step-by-step calls).
+   subroutine plbm_perform_dugks_steps(grid, nsteps)
+      type(lattice_grid), intent(inout) :: grid
+      integer, intent(in) :: nsteps
+      integer :: itmp, s
+
+      if (associated(grid%collision, plbm_grid_dugks_collide) .and. &
+          associated(grid%streaming, plbm_grid_dugks_stream)) then
+         call require_device(grid)
+         call plbm_check(plbm_dugks_step(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), int(nsteps,c_int), grid%dt, grid%omega, &
+            1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), "plbm_perform_dugks_steps")
+         call device_written(grid)
+         if (mod(nsteps, 2) == 1) then
+            itmp = grid%iold
+            grid%iold = grid%inew
+            grid%inew = itmp
+         end if
+      else
+         do s = 1, nsteps
+            call plbm_grid_perform_dugks_step(grid)
+         end do
+      end if
    end subroutine
 
    ! ------------------------------------------------------------------ output

@@ -439,6 +439,22 @@ def perform_dugks_step(grid: LatticeGrid) -> None:
     grid.iold, grid.inew = grid.inew, grid.iold
 
 
+def perform_dugks_steps(grid: LatticeGrid, nsteps: int) -> None:
+    """Engine extension: the effect of `nsteps` perform_dugks_step calls in one call; with
+    dugks_collide + dugks_stream in the slots all steps but the last run in temporally
+    blocked passes (bit-identical, same slot contents)."""
+    if grid.collision is dugks_collide and grid.streaming is dugks_stream:
+        _require_device(grid)
+        grid.lattice.dugks_step(grid.iold - 1, grid.inew - 1, nsteps, grid.dt, grid.omega,
+                                _dugks_omega_half(grid))
+        _device_written(grid)
+        if nsteps % 2:
+            grid.iold, grid.inew = grid.inew, grid.iold
+    else:
+        for _ in range(nsteps):
+            perform_dugks_step(grid)
+
+
 _ENGINE_COLLISIONS = {collide_bgk: _abi.BGK, collide_trt: _abi.TRT, collide_rr: _abi.RR,
                       collide_bgk_split: _abi.BGK_SPLIT, dugks_collide: None}
 _ENGINE_STREAMING = (lbm_stream, stream_fvm_bardow, stream_fdm_bardow, stream_fdm_sofonea,

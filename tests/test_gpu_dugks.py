@@ -116,6 +116,34 @@ def test_dugks_golden(P, gpu, nx, ny, dt):
         P.dealloc_grid(g)
 
 
+@pytest.mark.parametrize("nsteps", [6, 7])
+def test_dugks_perform_steps(P, gpu, nsteps):
+    """perform_dugks_steps(grid, n) == n x perform_dugks_step (both slots, ids, macros),
+    in passes of 3 on a 300 x 200 grid."""
+    nx, ny, dt = 300, 200, 0.8
+    rho, ux, uy = fields(nx, ny, 21)
+    grids = []
+    for _ in range(2):
+        g = P.alloc_grid(nx, ny, 2, log=False)
+        g.mf[:, :, 0], g.mf[:, :, 1], g.mf[:, :, 2] = rho, ux, uy
+        P.set_properties(g, 0.01, dt, 0.25)
+        g.collision, g.streaming = P.dugks_collide, P.dugks_stream
+        P.set_pdf_to_equilibrium(g)
+        grids.append(g)
+    grids[0].lattice.dugks_set_steps_per_pass(3)
+    P.perform_dugks_steps(grids[0], nsteps)
+    for _ in range(nsteps):
+        P.perform_dugks_step(grids[1])
+    a, b = grids
+    assert (a.iold, a.inew) == (b.iold, b.inew)
+    P.sync_host(a)
+    P.sync_host(b)
+    np.testing.assert_array_equal(a.f[:ny], b.f[:ny])
+    P.update_macros(a)
+    P.update_macros(b)
+    np.testing.assert_array_equal(a.mf, b.mf)
+
+
 def test_dugks_errors(P, gpu):
     lat = P.Lattice(32, 32, 2, "f32s")
     with pytest.raises(P.PlbmError):
@@ -129,6 +157,8 @@ def test_dugks_errors(P, gpu):
         lat.dugks_step(0, 1, -1, 1.0, 1.5, 1.5)
     with pytest.raises(P.PlbmError):
         lat.dugks_step(0, 1, 1, float("nan"), 1.5, 1.5)
+    with pytest.raises(P.PlbmError):
+        lat.dugks_set_steps_per_pass(4)
 
 
 def test_dugks_full_size_bit_exact_4096(P, gpu, oracle):
