@@ -171,9 +171,10 @@ def schemes_leg(P, torch, args, stream, fields):
     """The other schemes on the same 8192^2 FP64 start: DUGKS (src/periodic_dugks.F90) as
     the fused two-kernel step and as the reference's collide + stream sequence, and the
     finite-volume / finite-difference streaming operators (src/fvm_bardow.F90:393-893)
-    with collide_bgk as perform_step runs them (one fused pass).  Algorithmic bytes per
-    cell and step: fused DUGKS 360 (collide: read 72 + write 72; flux: read ft 72 + f 72,
-    write 72), reference-form DUGKS 432, stream operator + BGK 144."""
+    with collide_bgk as perform_step runs them (one fused pass per step), and the same
+    pair through plbm_run_op (temporally blocked passes).  Algorithmic bytes per cell and
+    step: fused DUGKS 360 (collide: read 72 + write 72; flux: read ft 72 + f 72, write
+    72), reference-form DUGKS 432, stream operator + BGK 144 (per pass for run_op)."""
     n = args.n
     cells = n * n
     tau = 3.0 * NU
@@ -217,6 +218,9 @@ def schemes_leg(P, torch, args, stream, fields):
             for s in range(k):
                 lat.step_op(op, "bgk", s % 2, 1 - s % 2, 1.0)
         out[f"stream_{op}+bgk"] = rec(timed(run, steps, 2), 144)
+        out[f"stream_{op}+bgk_passes"] = rec(
+            timed(lambda k, op=op: lat.run_op(op, "bgk", k, 0, 1, 1.0), steps, 4), 144)
+        out[f"stream_{op}+bgk_passes"]["steps_per_pass"] = lat.get_op_steps_per_pass(op, "bgk")
     lat.close()
     torch.cuda.empty_cache()
     return out

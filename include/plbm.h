@@ -150,6 +150,18 @@ int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
  * unfused path stores the intermediate exactly); in FP32/FP16 storage it skips that
  * intermediate rounding.  op = PLBM_STREAM_EXACT is plbm_step.  Asynchronous. */
 int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt);
+/* nsteps x perform_step with streaming operator `op` and collide_<model> (the fused pair),
+ * with the reference's buffer bookkeeping as plbm_run (each step reads *iold, writes
+ * *inew, swaps them).  In FP64 storage all steps but the last run in temporally blocked
+ * passes of up to plbm_set_op_steps_per_pass steps (bit-identical to single steps; the
+ * two slots may exchange their device storage, as in plbm_run).  Asynchronous. */
+int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *iold, int *inew,
+                double dt);
+/* Pass depth of plbm_run_op: 1 = single fused steps, 2..3 = passes (FP64 storage only),
+ * 0 = chosen from the grid size, operator and model.  The get variant reports the
+ * effective value for (op, model). */
+int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps);
+int plbm_get_op_steps_per_pass(const plbm_lattice *h, int op, int model, int *steps);
 
 /* DUGKS, src/periodic_dugks.F90 (default build, without -DDUGKS), FP64 storage only
  * (PLBM_ERR_STATE otherwise).  omega_full is grid%omega, omega_half the reference's

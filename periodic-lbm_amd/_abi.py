@@ -107,6 +107,9 @@ SIGNATURES = {
     "plbm_field_band_limited": (_i, [_i, _i, _i, _d, _u64, _dp, _dp]),
     "plbm_stream_op": (_i, [_vp, _i, _i, _i, _d]),
     "plbm_step_op": (_i, [_vp, _i, _i, _i, _i, _d]),
+    "plbm_run_op": (_i, [_vp, _i, _i, _i, _ip, _ip, _d]),
+    "plbm_set_op_steps_per_pass": (_i, [_vp, _i]),
+    "plbm_get_op_steps_per_pass": (_i, [_vp, _i, _i, _ip]),
     "plbm_set_steps_per_launch": (_i, [_vp, _i, _i]),
     "plbm_pass": (_i, [_vp, _i, _i, _i, _i]),
     "plbm_get_steps_per_launch": (_i, [_vp, _i, _ip]),

@@ -154,6 +154,7 @@ struct plbm_lattice {
     plbm::LaunchCfg cfg[plbm::kModels];  // fused-step launch shape per collision model
     bool tuned[plbm::kModels] = {};
     int dugks_ns = 0;                    // DUGKS steps per pass (0 = chosen from the grid)
+    int fd_ns = 0;                       // FV/FD operator steps per pass of plbm_run_op
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

@@ -143,24 +143,6 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Row y+1 / y-1 of a value held by every lane of a wavefront whose lane i holds row
-// base + i: DPP wave_shl:1 gives lane i the value of lane i+1, wave_shr:1 that of lane
-// i-1 (the edge lanes keep their own value; they are halo lanes of a pass).
-__device__ __forceinline__ double row_plus(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x130, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x130, 0xF, 0xF,
-                                               false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ double row_minus(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x138, 0xF, 0xF,
-                                               false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // NS DUGKS steps per sweep over device memory (temporal blocking, as the pass kernel of
 // the lattice Boltzmann step): a wavefront owns 64 consecutive rows (56 with output, NS
 // halo rows on each side) and walks a strip of kMultiXS columns.  For every level k < NS

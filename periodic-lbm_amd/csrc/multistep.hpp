@@ -24,6 +24,24 @@ constexpr int max_steps_per_pass() {
     return (64 - multi_vr<ST>()) / 2;
 }
 
+// Row y+1 / y-1 of a value held by every lane of a wavefront whose lane i holds row
+// base + i: DPP wave_shl:1 gives lane i the value of lane i+1, wave_shr:1 that of lane
+// i-1 (the edge lanes keep their own value; they are halo lanes of a pass).
+__device__ __forceinline__ double row_plus(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x130, 0xF, 0xF,
+                                               false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double row_minus(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x138, 0xF, 0xF,
+                                               false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1
 template <class R>
 __device__ __forceinline__ R pull_y(int q, R v) {

@@ -13,8 +13,12 @@
 // and y over nx (square grids); here x runs over nx and y over ny.
 #include <cmath>
 
+#include <algorithm>
+#include <utility>
+
 #include "common.hpp"
 #include "d2q9.hpp"
+#include "multistep.hpp"
 #include "stepper.hpp"
 
 namespace plbm {
@@ -131,6 +135,80 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// NS steps of (streaming operator OP, collision MODEL) per sweep over device memory, FP64
+// storage (temporal blocking as the lattice Boltzmann pass kernel): a wavefront owns 64
+// consecutive rows (56 with output, NS halo rows on each side) and walks a strip of
+// kMultiXS columns, keeping for every level k < NS the last three columns of its own row
+// in registers; y-neighbours come from the adjacent lanes (DPP).  When level k receives a
+// column, level k+1 gets the column before it: the operator's 3x3 stencil per direction,
+// then the collision, exactly as k_stream_fd<OP, F64, MODEL> computes them.
+template <int OP, int MODEL, int NS>
+__global__ void __launch_bounds__(256)
+    k_stream_fd_multi(StepArgs a, int waves_y, int nwaves, double dt, double p2s) {
+    constexpr int VR = 56;
+    static_assert(VR + 2 * NS <= 64, "halo rows do not fit in a wavefront");
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= nwaves) return;  // the whole wavefront
+    const int nx = a.nx, ny = a.ny;
+    const int wy = gw % waves_y, xsi = gw / waves_y;
+    const int r = VR * wy - NS + lane;
+    int y = r % ny;
+    if (y < 0) y += ny;
+    const bool out = lane >= NS && lane < VR + NS && r < ny;
+    const int x0 = xsi * kMultiXS;
+    const int ncols = min(kMultiXS, nx - x0);
+    const size_t L = (size_t)a.ldd, cs = a.cs;
+    const double *__restrict__ s = static_cast<const double *>(a.src);
+    double *__restrict__ d = static_cast<double *>(a.dst);
+    auto column = [&](int c) {
+        c %= nx;
+        if (c < 0) c += nx;
+        return (size_t)c * cs;
+    };
+    double W[NS][3][9];  // [level][newest column, -1, -2][q], own row
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) W[k][j][q] = 0.0;
+    const int iters = ncols + 2 * NS;
+    for (int i = 0; i < iters; ++i) {
+        double f[9];
+        {
+            const size_t cc = column(x0 - NS + i);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) f[q] = s[cc + q * L + y];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            if (i < 2 * k) break;  // level k has no column yet
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                W[k][2][q] = W[k][1][q];
+                W[k][1][q] = W[k][0][q];
+                W[k][0][q] = f[q];
+            }
+            if (i < 2 * k + 2) break;  // level k+1 needs three columns of level k
+            f[0] = W[k][1][0];
+#pragma unroll
+            for (int q = 1; q < 9; ++q) {
+                const double fc = W[k][1][q], fe = W[k][0][q], fw = W[k][2][q];
+                f[q] = fd_update<OP, double>(q, fc, fe, row_plus(fc), fw, row_minus(fc),
+                                             row_plus(fe), row_plus(fw), row_minus(fw),
+                                             row_minus(fe), dt, p2s);
+            }
+            collide_cell<MODEL, PLBM_F64>(f, a);
+            if (k + 1 == NS && out) {
+                const size_t co = (size_t)(x0 - 2 * NS + i) * cs;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) __builtin_nontemporal_store(f[q], d + co + q * L + y);
+            }
+        }
+    }
+}
+
 }  // namespace plbm
 
 using namespace plbm;
@@ -156,9 +234,9 @@ StepArgs fd_args(const plbm_lattice *h, const void *src, void *dst) {
 }
 
 // model < 0: streaming only
-int launch_fd(plbm_lattice *h, int op, int model, int src, int dst, double dt) {
+int launch_fd(plbm_lattice *h, int op, int model, const void *src, void *dst, double dt) {
     const double p2s = 0.5 / std::sqrt(2.0);  // src/fvm_bardow.F90:712
-    const StepArgs a = fd_args(h, h->f[src], h->f[dst]);
+    const StepArgs a = fd_args(h, src, dst);
     const int xs = strip_columns(h->nx, h->ny);
     const dim3 grid((unsigned)a.nby, (unsigned)((h->nx + xs - 1) / xs));
     return dispatch_storage(h->storage, [&](auto S) {
@@ -210,12 +288,40 @@ int check_op_args(plbm_lattice *h, int op, int src, int dst, double dt) {
     return PLBM_OK;
 }
 
+int launch_fd_multi(plbm_lattice *h, int op, int model, int ns, const void *src, void *dst,
+                    double dt) {
+    const double p2s = 0.5 / std::sqrt(2.0);  // src/fvm_bardow.F90:712
+    const StepArgs a = fd_args(h, src, dst);
+    const int waves_y = (h->ny + 55) / 56;
+    const int nw = waves_y * ((h->nx + kMultiXS - 1) / kMultiXS);
+    const dim3 grid((unsigned)((nw + 3) / 4));
+    return dispatch_model(model, [&](auto M) {
+        constexpr int MM = decltype(M)::value;
+        auto go = [&](auto O) {
+            constexpr int OO = decltype(O)::value;
+            if (ns == 2)
+                hipLaunchKernelGGL((k_stream_fd_multi<OO, MM, 2>), grid, dim3(256), 0, h->stream,
+                                   a, waves_y, nw, dt, p2s);
+            else
+                hipLaunchKernelGGL((k_stream_fd_multi<OO, MM, 3>), grid, dim3(256), 0, h->stream,
+                                   a, waves_y, nw, dt, p2s);
+        };
+        switch (op) {
+        case PLBM_STREAM_FVM_BARDOW: go(std::integral_constant<int, PLBM_STREAM_FVM_BARDOW>{}); break;
+        case PLBM_STREAM_FDM_BARDOW: go(std::integral_constant<int, PLBM_STREAM_FDM_BARDOW>{}); break;
+        default: go(std::integral_constant<int, PLBM_STREAM_FDM_SOFONEA>{});
+        }
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    });
+}
+
 }  // namespace
 
 extern "C" int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt) {
     PLBM_TRY(check_op_args(h, op, src, dst, dt));
     if (op == PLBM_STREAM_EXACT) return plbm_stream(h, src, dst);
-    return launch_fd(h, op, -1, src, dst, dt);
+    return launch_fd(h, op, -1, h->f[src], h->f[dst], dt);
 }
 
 extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt) {
@@ -224,5 +330,73 @@ extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst
                  "unknown collision model " + std::to_string(model));
     PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE, "plbm_set_params has not been called");
     if (op == PLBM_STREAM_EXACT) return plbm_step(h, model, src, dst);
-    return launch_fd(h, op, model, src, dst, dt);
+    return launch_fd(h, op, model, h->f[src], h->f[dst], dt);
+}
+
+// Steps per pass of plbm_run_op when the caller has not chosen, where the grid gives at
+// least 2048 wavefronts (FP64 storage; else single steps): 3, except 2 for the 9-point
+// operators with the TRT / RR collisions, whose 3-step pass no longer fits two
+// wavefronts per SIMD (measured at 8192^2: FVM + TRT 1.49 ms per step in passes of 2,
+// 1.83 in passes of 3, 2.09 in single steps).
+static int fd_steps_per_pass(const plbm_lattice *h, int op, int model) {
+    if (h->fd_ns > 0) return h->fd_ns;
+    if (h->storage != PLBM_F64) return 1;
+    const long waves = (long)((h->ny + 55) / 56) * ((h->nx + kMultiXS - 1) / kMultiXS);
+    if (waves < 2048) return 1;
+    if (op == PLBM_STREAM_FDM_SOFONEA || model == PLBM_BGK || model == PLBM_BGK_SPLIT) return 3;
+    return 2;
+}
+
+extern "C" int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(steps >= 0 && steps <= 3, PLBM_ERR_ARG, "steps per pass must be 0..3");
+    PLBM_REQUIRE(steps <= 1 || h->storage == PLBM_F64, PLBM_ERR_STATE,
+                 "passes of the streaming operators need FP64 (PLBM_F64) storage");
+    h->fd_ns = steps;
+    return PLBM_OK;
+}
+
+extern "C" int plbm_get_op_steps_per_pass(const plbm_lattice *h, int op, int model,
+                                          int *steps) {
+    PLBM_REQUIRE(h && steps, PLBM_ERR_ARG, "null argument");
+    *steps = fd_steps_per_pass(h, op, model);
+    return PLBM_OK;
+}
+
+extern "C" int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *iold,
+                           int *inew, double dt) {
+    PLBM_REQUIRE(h && iold && inew, PLBM_ERR_ARG, "null argument");
+    PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
+    PLBM_TRY(check_op_args(h, op, *iold, *inew, dt));
+    PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
+                 "unknown collision model " + std::to_string(model));
+    PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE, "plbm_set_params has not been called");
+    if (op == PLBM_STREAM_EXACT) return plbm_run(h, model, nsteps, iold, inew);
+    const int ns = fd_steps_per_pass(h, op, model);
+    if (ns <= 1 || nsteps < 2) {
+        for (int s = 0; s < nsteps; ++s) {
+            PLBM_TRY(launch_fd(h, op, model, h->f[*iold], h->f[*inew], dt));
+            std::swap(*iold, *inew);
+        }
+        return PLBM_OK;
+    }
+    // all steps but the last in passes; the last a single fused step, so that the other
+    // slot ends with the state one step earlier (as after perform_step)
+    const int a0 = *iold, b0 = *inew;
+    void *latest = h->f[a0], *other = h->f[b0];
+    for (int done = 0; done < nsteps - 1;) {
+        const int k = std::min(ns, nsteps - 1 - done);
+        if (k == 1) PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+        else PLBM_TRY(launch_fd_multi(h, op, model, k, latest, other, dt));
+        std::swap(latest, other);
+        done += k;
+    }
+    PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+    std::swap(latest, other);
+    // reference parity: after nsteps swaps the latest state is in slot b0 when nsteps is odd
+    const int want = (nsteps % 2) ? b0 : a0;
+    if (h->f[want] != latest) std::swap(h->f[a0], h->f[b0]);
+    *iold = want;
+    *inew = want == a0 ? b0 : a0;
+    return PLBM_OK;
 }

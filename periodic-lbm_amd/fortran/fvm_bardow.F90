@@ -445,11 +445,12 @@ contains
          "plbm_set_steps_per_pass")
    end subroutine
 
-   !> Engine extension: the effect of `nsteps` calls of perform_step in one call.  With
-   !> lbm_stream and an engine collision in the slots the steps run in temporally blocked
-   !> passes of several steps per sweep over device memory (plbm_run; bit-identical to
-   !> single steps in FP64 storage, the slots end as after nsteps perform_step calls);
-   !> any other pair is stepped one perform_step at a time.
+   !> Engine extension: the effect of `nsteps` calls of perform_step in one call.  With an
+   !> engine streaming operator (lbm_stream, stream_fvm_bardow, stream_fdm_bardow,
+   !> stream_fdm_sofonea) and an engine collision in the slots the steps run in temporally
+   !> blocked passes of several steps per sweep over device memory (plbm_run_op;
+   !> bit-identical to single steps in FP64 storage, the slots end as after nsteps
+   !> perform_step calls); any other pair is stepped one perform_step at a time.
    subroutine plbm_perform_steps(grid, nsteps)
       type(lattice_grid), intent(inout) :: grid
       integer, intent(in) :: nsteps
@@ -457,12 +458,12 @@ contains
       integer(c_int) :: iold, inew
 
       model = plbm_fused_model(grid)
-      if (model >= 0 .and. plbm_fused_op(grid) == PLBM_STREAM_EXACT) then
+      if (model >= 0) then
          call require_device(grid)
          iold = int(grid%iold - 1, c_int)
          inew = int(grid%inew - 1, c_int)
-         call plbm_check(plbm_run(grid%dev, int(model,c_int), int(nsteps,c_int), iold, inew), &
-            "plbm_perform_steps")
+         call plbm_check(plbm_run_op(grid%dev, int(plbm_fused_op(grid),c_int), &
+            int(model,c_int), int(nsteps,c_int), iold, inew, grid%dt), "plbm_perform_steps")
          call device_written(grid)
          grid%iold = iold + 1
          grid%inew = inew + 1

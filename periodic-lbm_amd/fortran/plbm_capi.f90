@@ -15,7 +15,7 @@ module plbm_capi
 
    public :: plbm_create, plbm_destroy, plbm_set_params, plbm_upload_f, plbm_download_f
    public :: plbm_equilibrium_init, plbm_step, plbm_run, plbm_stream, plbm_collide, plbm_copy
-   public :: plbm_stream_op, plbm_step_op, plbm_set_steps_per_launch
+   public :: plbm_stream_op, plbm_step_op, plbm_set_steps_per_launch, plbm_run_op
    public :: plbm_dugks_collide, plbm_dugks_stream, plbm_dugks_step
    integer(c_int), parameter, public :: PLBM_STREAM_EXACT = 0, PLBM_STREAM_FVM_BARDOW = 1, &
                                         PLBM_STREAM_FDM_BARDOW = 2, PLBM_STREAM_FDM_SOFONEA = 3
@@ -93,6 +93,15 @@ module plbm_capi
          import :: c_int, c_ptr
          type(c_ptr), value :: h
          integer(c_int), value :: model, steps
+      end function
+
+      integer(c_int) function plbm_run_op(h, op, model, nsteps, iold, inew, dt) &
+            bind(c, name="plbm_run_op")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: op, model, nsteps
+         integer(c_int), intent(inout) :: iold, inew
+         real(c_double), value :: dt
       end function
 
       integer(c_int) function plbm_step_op(h, op, model, src, dst, dt) bind(c, name="plbm_step_op")

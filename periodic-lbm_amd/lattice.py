@@ -163,6 +163,25 @@ class Lattice:
         check(lib().plbm_step_op(self.h, _abi.STREAM_OPS.get(op, op),
                                  _abi.MODEL_IDS.get(model, model), src, dst, dt))
 
+    def run_op(self, op: str | int, model: str | int, nsteps: int, iold: int, inew: int,
+               dt: float = 1.0) -> tuple[int, int]:
+        """nsteps fused (streaming operator, collision) steps with the reference's slot
+        bookkeeping; FP64 storage runs them in temporally blocked passes."""
+        a, b = ctypes.c_int(iold), ctypes.c_int(inew)
+        check(lib().plbm_run_op(self.h, _abi.STREAM_OPS.get(op, op),
+                                _abi.MODEL_IDS.get(model, model), nsteps, ctypes.byref(a),
+                                ctypes.byref(b), dt))
+        return a.value, b.value
+
+    def set_op_steps_per_pass(self, steps: int):
+        check(lib().plbm_set_op_steps_per_pass(self.h, steps))
+
+    def get_op_steps_per_pass(self, op: str | int = "fvm_bardow", model: str | int = "bgk") -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_get_op_steps_per_pass(self.h, _abi.STREAM_OPS.get(op, op),
+                                               _abi.MODEL_IDS.get(model, model), ctypes.byref(v)))
+        return v.value
+
     def collide(self, model: str | int, buf: int):
         check(lib().plbm_collide(self.h, _abi.MODEL_IDS.get(model, model), buf))
 
@@ -491,14 +510,17 @@ perform_lbm_step = perform_step
 
 
 def perform_steps(grid: LatticeGrid, nsteps: int) -> None:
-    """Engine extension: the effect of `nsteps` perform_step calls in one call.  With
-    lbm_stream and an engine collision the steps run in temporally blocked passes
-    (Lattice.run; bit-identical to single steps in FP64 storage, same slot contents);
-    any other pair is stepped one perform_step at a time."""
+    """Engine extension: the effect of `nsteps` perform_step calls in one call.  With an
+    engine streaming operator (lbm_stream or a FV/FD operator) and an engine collision the
+    steps run in temporally blocked passes (Lattice.run / run_op; bit-identical to single
+    steps in FP64 storage, same slot contents); any other pair is stepped one
+    perform_step at a time."""
     model = _ENGINE_COLLISIONS.get(grid.collision)
-    if model is not None and grid.streaming is lbm_stream:
+    op = _FUSABLE_STREAMING.get(grid.streaming)
+    if model is not None and op is not None:
         _require_device(grid)
-        iold, inew = grid.lattice.run(model, nsteps, grid.iold - 1, grid.inew - 1)
+        iold, inew = grid.lattice.run_op(op, model, nsteps, grid.iold - 1, grid.inew - 1,
+                                         grid.dt)
         _device_written(grid)
         grid.iold, grid.inew = iold + 1, inew + 1
     else:

@@ -158,3 +158,45 @@ def test_unchanged_vortex_app(gpu, tmp_path):
     for name in ("results000000000.vtk", "results000000000.npy"):
         got = hashlib.sha256((tmp_path / "vortex" / name).read_bytes()).hexdigest()
         assert got == ref["files"][name], name
+
+
+@pytest.mark.parametrize("per_pass", [2, 3])
+@pytest.mark.parametrize("model", ["bgk", "trt", "rr"])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("nx,ny,nsteps", [(200, 130, 7), (64, 64, 6), (5, 3, 5), (130, 1, 4)])
+def test_operator_passes_equal_single_steps(P, gpu, op, model, per_pass, nx, ny, nsteps):
+    """plbm_run_op in passes of 2 / 3 steps == single fused steps, both slots, FP64."""
+    print(f"case {op}+{model} passes of {per_pass} {nx}x{ny} n={nsteps}", flush=True)
+    rho, ux, uy = fields(nx, ny, nx + ny + nsteps)
+    out = []
+    for pp in (1, per_pass):
+        lat = P.Lattice(nx, ny, 2, "f64")
+        lat.set_params(1.0 / (3 * 0.01 / 0.8 + 0.5), 0.25)
+        lat.set_op_steps_per_pass(pp)
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run_op(op, model, nsteps, 0, 1, 0.8)
+        out.append((i, n, lat.download_f(i), lat.download_f(n)))
+    (i1, n1, a1, b1), (i2, n2, a2, b2) = out
+    assert (i1, n1) == (i2, n2) == ((1, 0) if nsteps % 2 else (0, 1))
+    np.testing.assert_array_equal(a1, a2)
+    np.testing.assert_array_equal(b1, b2)
+
+
+def test_operator_passes_full_size(P, gpu, oracle):
+    """4096^2, stream_fvm_bardow + BGK, 5 steps in passes (automatic depth) vs the oracle."""
+    n, nsteps, dt = 4096, 5, 1.0
+    print(f"case fvm passes {n}x{n}", flush=True)
+    rho, ux, uy = fields(n, n, 77)
+    omega = 1.0 / (3 * 0.01 + 0.5)
+    lat = P.Lattice(n, n, 2, "f64")
+    lat.set_params(omega, 0.25)
+    assert lat.get_op_steps_per_pass() > 1
+    lat.equilibrium_init(0, rho, ux, uy)
+    i, _ = lat.run_op("fvm_bardow", "bgk", nsteps, 0, 1, dt)
+    a, b = po.new_f(n, n), po.new_f(n, n)
+    oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, a)
+    for _ in range(nsteps):
+        oracle.stream_op("fvm_bardow", n, n, a, b, dt)
+        oracle.collide("bgk", n, n, b, omega)
+        a, b = b, a
+    np.testing.assert_array_equal(lat.download_f(i)[:n], a[:n])
