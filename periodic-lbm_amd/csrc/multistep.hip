@@ -12,8 +12,11 @@
 //    c - k - 1 follows from level k at columns c - k, c - k - 1, c - k - 2 (the pull
 //    f_q(x, y) <- f_q(x - cx_q, y - cy_q) takes x-neighbours from the window and
 //    y-neighbours from the adjacent lanes), and level NS is stored.
-//  * The next column's loads are issued one column ahead (the pass is latency-bound
-//    without it: each wavefront walks its strip sequentially).
+//  * The next columns' loads are issued ahead (the pass is latency-bound without it:
+//    each wavefront walks its strip sequentially): for 8-byte storage two columns ahead,
+//    straight into LDS (16-byte global -> LDS loads, two rows per lane), and the pull in
+//    y of level 0 reads the neighbouring row's LDS slot; for 4- and 2-byte storage one
+//    column ahead into registers.
 //  * Each level applies exactly the single-step kernel's per-cell arithmetic
 //    (collide_cell), and intermediate levels are rounded to the storage type, so the
 //    result equals NS launches of the single-step kernel bit for bit.

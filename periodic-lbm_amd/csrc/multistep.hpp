@@ -42,6 +42,36 @@ __device__ __forceinline__ double row_minus(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+using lds_void = __attribute__((address_space(3))) void;
+using gbl_void = __attribute__((address_space(1))) void;
+
+// 8-byte storage stages the next columns in LDS (direct global -> LDS loads into
+// kStageAhead + 1 buffers per wavefront) instead of registers, so that the loads in
+// flight cost no VGPRs.  A buffer holds the 64 rows of the 9 populations of one column,
+// plus one slot of padding, 512 bytes each.  Measured on the 8192^2 FP64 TRT pass of 4
+// steps: register prefetch of one column 2.01 ms (172 VGPRs, 2 waves per SIMD); staged,
+// one column ahead 2.00 ms (3 waves per SIMD); two ahead 1.95 ms (LDS limits it to 2
+// waves per SIMD, with twice the bytes in flight); three ahead 1.96 ms.  Four wavefronts
+// per workgroup (the pass kernels launch 256 threads).
+constexpr int kStageWaves = 4;
+constexpr int kStageAhead = 2;  // columns in flight ahead of the one being consumed
+constexpr int kStageBufs = kStageAhead + 1;
+constexpr int kStageSlots = 10;
+
+// Wait until at most n vector memory operations of this wavefront are outstanding, n
+// rounded down to one of the counts the staged pass produces (rounding down is a stronger
+// wait): vmcnt takes an immediate.
+__device__ __forceinline__ void wait_vm_at_most(int n) {
+    if (n >= 36) asm volatile("s_waitcnt vmcnt(36)" ::: "memory");
+    else if (n >= 27) asm volatile("s_waitcnt vmcnt(27)" ::: "memory");
+    else if (n >= 23) asm volatile("s_waitcnt vmcnt(23)" ::: "memory");
+    else if (n >= 18) asm volatile("s_waitcnt vmcnt(18)" ::: "memory");
+    else if (n >= 14) asm volatile("s_waitcnt vmcnt(14)" ::: "memory");
+    else if (n >= 9) asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
+    else if (n >= 5) asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
 // value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1
 template <class R>
 __device__ __forceinline__ R pull_y(int q, R v) {
@@ -84,25 +114,115 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int xsi) {
     };
     // the next column's loads are issued one iteration ahead, so that their latency
     // overlaps the collisions of the current column
+    constexpr bool kStage = sizeof(T) == 8;
     T nxt[9];
-    {
-        const size_t cc = column(x0 - NS);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) nxt[q] = s[cc + q * L + y];
+    // Staged path.  When the wavefront's first row and ny are even, no pair of rows
+    // (2m, 2m+1) of the wavefront straddles the periodic wrap, and one 16-byte load per
+    // lane moves two rows: lanes 0-31 fill the slot of population q, lanes 32-63 that of
+    // q + 1 (5 loads per column; the fifth moves population 8 twice, the second copy into
+    // the padding slot).  Otherwise lane j moves the 4-byte half (j & 1) of row (j >> 1)
+    // and then of row 32 + (j >> 1) of each population (18 loads per column).
+    unsigned stage_a = 0, stage_b = 0;
+    bool pairs = false;
+    double *stage = nullptr;
+    if constexpr (kStage) {
+        __shared__ double stage_mem[kStageWaves][kStageBufs][kStageSlots][64];
+        const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & (kStageWaves - 1);
+        stage = &stage_mem[wib][0][0][0];
+        auto row_of = [&](int m) {
+            int yy = (VR * wy - NS + m) % ny;
+            return yy < 0 ? yy + ny : yy;
+        };
+        pairs = (NS % 2 == 0) && (ny % 2 == 0);
+        if (pairs) {
+            const unsigned off = (unsigned)row_of(2 * (lane & 31)) * 8u;
+            stage_a = off + (lane >> 5) * (unsigned)(L * 8);  // populations q, q + 1
+            stage_b = off;                                     // population 8 twice
+        } else {
+            stage_a = (unsigned)row_of(lane >> 1) * 8u + (lane & 1) * 4u;
+            stage_b = (unsigned)row_of(32 + (lane >> 1)) * 8u + (lane & 1) * 4u;
+        }
     }
+    const int stage_loads = pairs ? 5 : 18;  // vector memory operations per staged column
+    auto fetch = [&](int c, int buf) {
+        const size_t cc = column(c);
+        if constexpr (kStage) {
+            double *slots = stage + buf * kStageSlots * 64;
+            if (pairs) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const char *plane = reinterpret_cast<const char *>(s + cc + 2 * m * L);
+                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_a),
+                                                     (lds_void *)(slots + 2 * m * 64), 16, 0, 0);
+                }
+                const char *plane = reinterpret_cast<const char *>(s + cc + 8 * L);
+                __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_b),
+                                                 (lds_void *)(slots + 8 * 64), 16, 0, 0);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const char *plane = reinterpret_cast<const char *>(s + cc + q * L);
+                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_a),
+                                                     (lds_void *)(slots + q * 64), 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_b),
+                                                     (lds_void *)(slots + q * 64 + 32), 4, 0, 0);
+                }
+            }
+            // keep the stores of this iteration behind these loads in issue order (the
+            // wait below counts on it)
+            asm volatile("" ::: "memory");
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) nxt[q] = s[cc + q * L + y];
+        }
+    };
     const int iters = ncols + 2 * NS;
+    constexpr int ahead = kStage ? kStageAhead : 1;
+#pragma unroll
+    for (int j = 0; j < ahead; ++j)
+        if (j < iters) fetch(x0 - NS + j, j);
     for (int i = 0; i < iters; ++i) {
+        if constexpr (kStage) {
+            // The previous column's LDS reads are complete before the staging loads
+            // below overwrite the buffer they read.
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            // Vector memory operations complete in issue order.  The prologue issues the
+            // staging loads of each of columns 0 .. ahead-1; iteration t those of column
+            // t + ahead (if it exists) and then its 9 stores (if t >= 2 NS).  Count what
+            // was issued after the loads of column i.
+            int after = 0, t0;
+            if (i < ahead) {
+                for (int j = i + 1; j < ahead; ++j) after += j < iters ? stage_loads : 0;
+                t0 = 0;
+            } else {
+                after += i - ahead >= 2 * NS ? 9 : 0;
+                t0 = i - ahead + 1;
+            }
+            for (int t = t0; t < i; ++t) {
+                after += t + ahead < iters ? stage_loads : 0;
+                after += t >= 2 * NS ? 9 : 0;
+            }
+            wait_vm_at_most(after);
+            const double *slot = stage + (i % kStageBufs) * kStageSlots * 64;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            W[0][2][q] = W[0][1][q];
-            W[0][1][q] = W[0][0][q];
-            W[0][0][q] = pull_y(q, load_pop<ST>(&nxt[q], a.inv_scale));
-        }
-        if (i + 1 < iters) {
-            const size_t cn = column(x0 - NS + i + 1);
+            for (int q = 0; q < 9; ++q) {
+                W[0][2][q] = W[0][1][q];
+                W[0][1][q] = W[0][0][q];
+                // the pull in y reads the neighbouring row's slot (edge lanes are halo)
+                const int src_lane = cy(q) > 0 ? max(lane - 1, 0)
+                                               : (cy(q) < 0 ? min(lane + 1, 63) : lane);
+                W[0][0][q] = load_pop<ST>(reinterpret_cast<const T *>(slot + q * 64 + src_lane),
+                                          a.inv_scale);
+            }
+        } else {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) nxt[q] = s[cn + q * L + y];
+            for (int q = 0; q < 9; ++q) {
+                W[0][2][q] = W[0][1][q];
+                W[0][1][q] = W[0][0][q];
+                W[0][0][q] = pull_y(q, load_pop<ST>(&nxt[q], a.inv_scale));
+            }
         }
+        if (i + ahead < iters) fetch(x0 - NS + i + ahead, (i + ahead) % (kStage ? kStageBufs : 2));
 #pragma unroll
         for (int k = 1; k <= NS; ++k) {
             if (i < 2 * k) break;
