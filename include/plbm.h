@@ -238,8 +238,8 @@ int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap);
 int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap);
 /* Temporal blocking of plbm_run: steps advanced per pass over HBM (1 = the single-step
  * kernel; 2..4 for F64 / F64S storage or 2..8 for F32S / F16S = that many fused steps
- * per launch, each wavefront walking a strip of columns with the intermediate states in
- * registers; 0 = chosen from the grid size: 4 when the grid gives at least 2048
+ * per launch, each wavefront walking a segment of columns with the intermediate states
+ * in registers; 0 = chosen from the grid size: 4 when the grid gives at least 2048
  * wavefronts, else 1).  In F64 storage the results are bit-identical for every setting;
  * the shifted storages agree to roundoff (their collisions allow FMA contraction, which
  * the compiler places per kernel).  plbm_run keeps the reference's slot contents (latest
@@ -252,6 +252,12 @@ int plbm_set_steps_per_launch(plbm_lattice *h, int model, int steps);
  * the building block of plbm_run.  Asynchronous. */
 int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst);
 int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *steps);
+/* Work split of a pass (engine extension, no counterpart in the reference): each band of
+ * 56 (F64 / F64S) or 48 (F32S / F16S) rows is cut into `segments` column ranges of nearly
+ * equal length, one wavefront each; 0 = chosen from the grid (64-column segments, longer
+ * on large grids).  Results do not depend on it.  The get variant reports the setting. */
+int plbm_set_pass_segments(plbm_lattice *h, int segments);
+int plbm_get_pass_segments(const plbm_lattice *h, int *segments);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
  * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
  * of 2..4 (2..8) steps against it per lattice update; keep the fastest for this lattice and

@@ -135,7 +135,8 @@ __global__ void __launch_bounds__(256)
     StepArgs a = case_args<ST>(c, scale, inv_scale);
     a.src = parity ? base + c.elems : base;
     a.dst = parity ? base : base + c.elems;
-    multi_wave<MODEL, ST, NS>(a, w.y0, w.x0);
+    const int x0 = w.x0 * kMultiXS;
+    multi_wave<MODEL, ST, NS>(a, w.y0, x0, min(kMultiXS, c.nx - x0));
 }
 
 // nsteps steps of one small case per workgroup; only workgroup barriers between steps.

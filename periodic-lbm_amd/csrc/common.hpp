@@ -155,6 +155,7 @@ struct plbm_lattice {
     bool tuned[plbm::kModels] = {};
     int dugks_ns = 0;                    // DUGKS steps per pass (0 = chosen from the grid)
     int fd_ns = 0;                       // FV/FD operator steps per pass of plbm_run_op
+    int pass_segs = 0;                   // column segments per row band of a pass (0 = auto)
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

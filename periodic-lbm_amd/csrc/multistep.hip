@@ -4,9 +4,10 @@
 // lattice update), so it is bound by HBM at one step per pass.  Here one pass advances the
 // lattice NS steps (NS = 2..4) and reads / writes every population once per pass:
 //
-//  * A wavefront owns 64 consecutive rows y and walks a strip of XS columns along x.  Of
-//    its rows, VR produce output; NS rows on each side are halo (VR + 2 NS <= 64).  VR is
-//    a multiple of 8 so that each wave's stores start and end on 64-byte boundaries.
+//  * A wavefront owns 64 consecutive rows y and walks a segment of XS columns along x
+//    (64 columns, or longer on large grids).  Of its rows, VR produce output; NS rows on
+//    each side are halo (VR + 2 NS <= 64).  VR is a multiple of 8 so that each wave's
+//    stores start and end on 64-byte boundaries.
 //  * Level k (k = 0 .. NS-1) keeps the pull-ready values of f^{n+k} for the last three
 //    columns in registers.  Loading column c gives level 0 at c; level k+1 at column
 //    c - k - 1 follows from level k at columns c - k, c - k - 1, c - k - 2 (the pull
@@ -25,6 +26,8 @@
 // population) for NS lattice updates; the halo re-reads (64/VR in y, (XS+2 NS)/XS in x)
 // are the overhead.  Work per output cell and pass: NS collisions plus the halo ones.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 
@@ -34,11 +37,32 @@ namespace plbm {
 
 int launch_step(plbm_lattice *h, int model, const void *src, void *dst);
 
+// The pass covers waves_y row bands x nx columns; each band is cut into m column segments
+// of nearly equal length, and wavefront w walks segment w / waves_y of band w % waves_y,
+// so that the wavefronts running at the same time hold adjacent bands of the same columns
+// (their loads are adjacent in memory, and the rows two bands share are read while both
+// are in L2).
 template <int MODEL, int ST, int NS>
-__global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int nwaves) {
+__global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
     const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (gw >= nwaves) return;  // the whole wavefront
-    multi_wave<MODEL, ST, NS>(a, gw % waves_y, gw / waves_y);
+    if (gw >= waves_y * m) return;  // the whole wavefront
+    const int band = gw % waves_y, j = gw / waves_y;
+    const int x0 = (int)((long long)a.nx * j / m), x1 = (int)((long long)a.nx * (j + 1) / m);
+    if (x1 > x0) multi_wave<MODEL, ST, NS>(a, band, x0, x1 - x0);
+}
+
+// Wavefronts the GPU holds at once for a kernel of 256-thread workgroups.
+static int resident_waves(const void *kernel, int device) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, device});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return cache[{kernel, device}] = per_cu * cus * 4;
 }
 
 int max_steps_per_pass(int storage) { return storage_bytes(storage) == 8 ? 4 : 8; }
@@ -62,13 +86,23 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst)
             constexpr int MM = decltype(M)::value, SS = decltype(S)::value;
             const int vr = multi_vr<SS>();
             const int waves_y = (h->ny + vr - 1) / vr;
-            const int nw = waves_y * ((h->nx + kMultiXS - 1) / kMultiXS);
-            const dim3 grid((unsigned)((nw + 3) / 4));
             auto go = [&](auto K) {
                 constexpr int NS = decltype(K)::value;
-                if constexpr (NS <= max_steps_per_pass<SS>())
+                if constexpr (NS <= max_steps_per_pass<SS>()) {
+                    const void *kern = reinterpret_cast<const void *>(&k_multi<MM, SS, NS>);
+                    // Segments of kMultiXS columns, or fewer and longer ones where that still
+                    // gives 4 wavefronts per resident slot (8192^2 FP64 TRT, 4 steps: 64-column
+                    // strips 1.96 ms per pass, 149-column segments 1.92 ms; one segment per
+                    // resident slot, 630 columns, 1.96 ms).
+                    int m = (h->nx + kMultiXS - 1) / kMultiXS;
+                    const long long res = 4LL * resident_waves(kern, h->device);
+                    if (res > 0) m = (int)std::min<long long>(m, std::max(1LL, res / waves_y));
+                    if (h->pass_segs > 0) m = std::min(h->pass_segs, h->nx);
+                    const long long nw = (long long)waves_y * m;
+                    const dim3 grid((unsigned)((nw + 3) / 4));
                     hipLaunchKernelGGL((k_multi<MM, SS, NS>), grid, dim3(256), 0, h->stream, a,
-                                       waves_y, nw);
+                                       waves_y, m);
+                }
                 return NS <= max_steps_per_pass<SS>();
             };
             bool ok = false;
@@ -156,4 +190,17 @@ extern "C" int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst
                  "collision parameters not set: call plbm_set_params first");
     if (steps == 1) return launch_step(h, model, h->f[src], h->f[dst]);
     return launch_multi(h, model, steps, h->f[src], h->f[dst]);
+}
+
+extern "C" int plbm_set_pass_segments(plbm_lattice *h, int segments) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(segments >= 0, PLBM_ERR_ARG, "segments must be >= 0 (0 = automatic)");
+    h->pass_segs = segments;
+    return PLBM_OK;
+}
+
+extern "C" int plbm_get_pass_segments(const plbm_lattice *h, int *segments) {
+    PLBM_REQUIRE(h && segments, PLBM_ERR_ARG, "null argument");
+    *segments = h->pass_segs;
+    return PLBM_OK;
 }

@@ -80,11 +80,11 @@ __device__ __forceinline__ R pull_y(int q, R v) {
     return v;
 }
 
-// The calling wavefront advances rows [VR*wy, VR*wy + VR) x columns [64*xsi, 64*xsi + 64)
-// of the grid described by `a` (clipped to the grid) NS steps, a.src -> a.dst.  All 64
-// lanes must call it together (lane shuffles).
+// The calling wavefront advances rows [VR*wy, VR*wy + VR) (clipped to the grid) x columns
+// [x0, x0 + ncols) (0 <= x0, x0 + ncols <= nx) of the grid described by `a` NS steps,
+// a.src -> a.dst.  All 64 lanes must call it together (lane shuffles).
 template <int MODEL, int ST, int NS>
-__device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int xsi) {
+__device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
     using T = typename Traits<ST>::T;
     using R = typename Traits<ST>::R;
     constexpr int VR = multi_vr<ST>();
@@ -95,8 +95,6 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int xsi) {
     int y = r % ny;
     if (y < 0) y += ny;
     const bool out = lane >= NS && lane < VR + NS && r < ny;
-    const int x0 = xsi * kMultiXS;
-    const int ncols = min(kMultiXS, nx - x0);
     const size_t L = (size_t)a.ldd, cs = a.cs;
     const T *__restrict__ s = static_cast<const T *>(a.src);
     T *__restrict__ d = static_cast<T *>(a.dst);

@@ -109,6 +109,16 @@ class Lattice:
                                               ctypes.byref(v)))
         return v.value
 
+    def set_pass_segments(self, segments: int):
+        """Column segments per row band of a pass (one wavefront each); 0 automatic.
+        Results do not depend on it."""
+        check(lib().plbm_set_pass_segments(self.h, segments))
+
+    def get_pass_segments(self) -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_get_pass_segments(self.h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, model: str | int, buf: int = 0, reps: int = 10) -> float:
         """Time every launch shape of the fused step and the temporal-blocking depths; keep
         the fastest.  Returns ms per lattice-update step."""

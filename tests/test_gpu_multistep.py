@@ -115,6 +115,48 @@ def test_full_size_passes_bit_exact_4096(P, gpu, oracle):
     np.testing.assert_array_equal(lat.download_f(n_)[:n], prev[:n])
 
 
+@pytest.mark.parametrize("segments", [1, 3, 7])
+@pytest.mark.parametrize("nx,ny", [(200, 130), (5, 3), (70, 1)])
+def test_pass_segments_do_not_change_results(P, gpu, nx, ny, segments):
+    """The work split of a pass (whole-band walks, ragged cuts, one-column segments where
+    segments > nx) leaves FP64 results bit-identical to single steps."""
+    print(f"case segments {segments} {nx}x{ny}", flush=True)
+    for model in ("trt", "rr"):
+        ref, ri, rn = run(P, "f64", model, 1, nx, ny, 9)
+        rho, ux, uy = fields(nx, ny, 1)
+        lat = P.Lattice(nx, ny, 2, "f64")
+        lat.set_params(1.0 / (3 * 0.004 + 0.5), 0.25)
+        lat.set_steps_per_launch(model, 4)
+        lat.set_pass_segments(segments)
+        assert lat.get_pass_segments() == segments
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run(model, 9, 0, 1)
+        assert (i, n) == (ri, rn)
+        np.testing.assert_array_equal(lat.download_f(i), ref.download_f(ri))
+        np.testing.assert_array_equal(lat.download_f(n), ref.download_f(rn))
+    with pytest.raises(P.PlbmError):
+        lat.set_pass_segments(-1)
+
+
+def test_automatic_segments_large_grid(P, gpu):
+    """8192 x 4096 FP64 TRT, 5 steps: the automatic work split (about 110 segments of 74-75
+    columns per band, more than one strip of 64) against single steps."""
+    nx, ny, nsteps = 8192, 4096, 5
+    print(f"case segments auto {nx}x{ny}", flush=True)
+    rho, ux, uy = fields(nx, ny, 9)
+    out = []
+    for ns in (1, 4):
+        lat = P.Lattice(nx, ny, 2, "f64")
+        lat.set_params(1.0 / (3 * 0.01 + 0.5), 0.25)
+        lat.set_steps_per_launch("trt", ns)
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run("trt", nsteps, 0, 1)
+        out.append((i, lat.download_f(i)))
+        lat.close()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_autotune_picks_a_depth(P, gpu):
     n = 2048
     lat = P.Lattice(n, n, 2, "f64")
