@@ -16,7 +16,7 @@
 //
 //  * k_batch_multi (steps per pass ns > 1): the tiled cases advance ns steps per launch
 //    with the temporally blocked pass of the single lattice (multistep.hpp), one
-//    wavefront per 56 x 64 (FP64) or 48 x 64 (FP32 / FP16) block of a case, listed in a
+//    wavefront per 56 x 64 (FP64, up to 4 steps) or 48 x 64 block of a case, listed in a
 //    wave table in place of the tile table.
 //
 // Tile shape (ty, tx), steps per pass and the resident threshold are picked by the
@@ -307,7 +307,7 @@ static int build_plan(plbm_batch *b, const BatchCfg &cfg, BatchTile **d_tiles, i
             continue;
         }
         if (cfg.ns > 1) {  // wave table: {case, strip, row block}
-            const int vr = storage_bytes(b->storage) == 8 ? 56 : 48;
+            const int vr = multi_rows(storage_bytes(b->storage), cfg.ns);
             for (int xs = 0; xs * kMultiXS < k.nx; ++xs)
                 for (int wy = 0; wy * vr < k.ny; ++wy) tiles.push_back({c, xs, wy, 0});
         } else {

@@ -136,7 +136,7 @@ inline int dispatch_model(int m, F &&f) {
 
 constexpr int kMaxBuffers = 4;
 constexpr int kModels = 4;
-constexpr int kMaxStepsPerLaunch = 8;  // temporal-blocking depth of plbm_run (FP64: 4)
+constexpr int kMaxStepsPerLaunch = 8;  // temporal-blocking depth of plbm_run
 int max_steps_per_pass(int storage);  // PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT
 constexpr int kReduceBlocks = 1024;
 

@@ -343,7 +343,61 @@ __device__ inline void collide_rr_shifted(R g[9], R omega) {
     g[8] = geq[8] + R(WD) * om * (id - R(9) * (axxy - ayyx + axy));
 }
 
-// Conservation restoration for reduced-precision arithmetic.  In FP32 the weights
+// FP32 forms of BGK and TRT that conserve mass and momentum by construction.  In FP32 the
+// weights w_q and the TRT prefactors are not representable, so the operators above change
+// rho and j by a fixed relative amount (~1e-8) per cell and step, a bias that accumulates
+// over thousands of steps.  Both operators conserve rho and j exactly in exact arithmetic,
+// so the populations that close the three balances can be computed from the others
+// without changing the operator: BGK takes geq_1, geq_2 from the momentum balances and
+// geq_0 from the mass balance of the equilibrium; TRT takes the antisymmetric parts of
+// pairs (1,3), (2,4) from those of (5,7), (8,6) and the rest population from the
+// symmetric parts.  What remains is the unbiased rounding of each update, and the
+// measure-and-correct pass (restore_conservation below) is not needed.
+template <class R>
+__device__ inline void collide_bgk_shifted_conserving(R g[9], R omega) {
+#pragma clang fp contract(fast)
+    R drho, jx, jy;
+    moments_shifted(g, drho, jx, jy);
+    const R inv = R(1) / (R(1) + drho);
+    R geq[9];
+    equilibrium_shifted(drho, jx * inv, jy * inv, geq);
+    const R e57 = geq[5] - geq[7];
+    geq[1] = geq[3] + (jx - (e57 + (geq[8] - geq[6])));
+    geq[2] = geq[4] + (jy - (e57 + (geq[6] - geq[8])));
+    geq[0] = drho - (((geq[5] + geq[7]) + (geq[6] + geq[8])) +
+                     ((geq[1] + geq[3]) + (geq[2] + geq[4])));
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = g[q] + omega * (geq[q] - g[q]);
+}
+
+template <class R>
+__device__ inline void collide_trt_shifted_conserving(R g[9], R le, R lo) {
+#pragma clang fp contract(fast)
+    R drho, jx, jy;
+    moments_shifted(g, drho, jx, jy);
+    const R jx2 = jx * jx, jy2 = jy * jy;
+    const R dc = drho - R(1.5) * (jx2 + jy2);
+    const R les = R(0.5) * le, los = R(0.5) * lo;
+    const R jp = jx + jy, jm = jx - jy;
+    const R s57 = les * (g[5] + g[7] - R(FAC2) * jp * jp - R(T2X2) * dc);
+    const R a57 = los * (g[5] - g[7] - R(3.0 * T2X2) * jp);
+    const R s86 = les * (g[8] + g[6] - R(FAC2) * jm * jm - R(T2X2) * dc);
+    const R a86 = los * (g[8] - g[6] - R(3.0 * T2X2) * jm);
+    const R s24 = les * (g[2] + g[4] - R(FAC1) * jy2 - R(T1X2) * dc);
+    const R s13 = les * (g[1] + g[3] - R(FAC1) * jx2 - R(T1X2) * dc);
+    // j_x changes by -2 (a13 + a57 + a86), j_y by -2 (a24 + a57 - a86), the mass by
+    // o_0 - g_0 - 2 (s57 + s86 + s24 + s13)
+    const R a13 = -(a57 + a86), a24 = a86 - a57;
+    const R o0 = g[0] + R(2) * ((s57 + s86) + (s24 + s13));
+    const R o5 = g[5] - s57 - a57, o7 = g[7] - s57 + a57;
+    const R o8 = g[8] - s86 - a86, o6 = g[6] - s86 + a86;
+    const R o2 = g[2] - s24 - a24, o4 = g[4] - s24 + a24;
+    const R o1 = g[1] - s13 - a13, o3 = g[3] - s13 + a13;
+    g[0] = o0; g[1] = o1; g[2] = o2; g[3] = o3; g[4] = o4;
+    g[5] = o5; g[6] = o6; g[7] = o7; g[8] = o8;
+}
+
+// Conservation restoration for reduced-precision arithmetic (the RR operator in FP32).  In FP32 the weights
 // w_q (4/9, 1/9, 1/36) and the TRT/RR prefactors are not representable, so the
 // equilibrium's zeroth and first moments miss rho and j by a fixed relative amount
 // (~1e-8) in every cell and step; over thousands of steps that bias accumulates in the

@@ -65,7 +65,7 @@ static int resident_waves(const void *kernel, int device) {
     return cache[{kernel, device}] = per_cu * cus * 4;
 }
 
-int max_steps_per_pass(int storage) { return storage_bytes(storage) == 8 ? 4 : 8; }
+int max_steps_per_pass(int) { return kMaxPassSteps; }
 
 int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst) {
     StepArgs a{};
@@ -84,11 +84,11 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst)
     return dispatch_model(model, [&](auto M) {
         return dispatch_storage(h->storage, [&](auto S) {
             constexpr int MM = decltype(M)::value, SS = decltype(S)::value;
-            const int vr = multi_vr<SS>();
-            const int waves_y = (h->ny + vr - 1) / vr;
             auto go = [&](auto K) {
                 constexpr int NS = decltype(K)::value;
                 if constexpr (NS <= max_steps_per_pass<SS>()) {
+                    constexpr int vr = multi_vr<SS, NS>();
+                    const int waves_y = (h->ny + vr - 1) / vr;
                     const void *kern = reinterpret_cast<const void *>(&k_multi<MM, SS, NS>);
                     // Segments of kMultiXS columns, or fewer and longer ones where that still
                     // gives 4 wavefronts per resident slot (8192^2 FP64 TRT, 4 steps: 64-column
@@ -168,7 +168,7 @@ namespace plbm {
 // Steps per pass when the caller has not chosen: 4 where the grid gives at least 2048
 // wavefronts (8 per CU), else single steps (a small grid cannot fill the chip with strips).
 int default_steps_per_launch(const plbm_lattice *h) {
-    const int vr = storage_bytes(h->storage) == 8 ? 56 : 48;
+    const int vr = multi_rows(storage_bytes(h->storage), 4);
     const long waves = (long)((h->ny + vr - 1) / vr) * ((h->nx + kMultiXS - 1) / kMultiXS);
     return waves >= 2048 ? 4 : 1;
 }

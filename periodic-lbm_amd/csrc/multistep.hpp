@@ -11,17 +11,20 @@ namespace plbm {
 
 constexpr int kMultiXS = 64;  // columns per strip
 
-// Rows with output per wave: 56 for 8-byte storage (448-byte aligned runs), 48 for 4- and
-// 2-byte storage (192 / 96 bytes, 64- / 32-byte aligned).  The halo of a pass of NS steps
-// is NS rows on each side, so a pass can hold (64 - VR) / 2 steps: 4 for FP64 storage,
-// 8 for FP32 and FP16 storage (whose registers are half as wide).
-template <int ST>
+// Rows with output per wave (VR): a pass of NS steps needs NS halo rows on each side, so
+// VR <= 64 - 2 NS.  56 for 8-byte storage at up to 4 steps (448-byte runs, 64-byte
+// aligned), else 48 (384 / 192 / 96-byte runs), which holds up to 8 steps.
+constexpr int kMaxPassSteps = 8;
+__host__ __device__ constexpr int multi_rows(int storage_bytes, int ns) {
+    return storage_bytes == 8 && ns <= 4 ? 56 : 48;
+}
+template <int ST, int NS>
 constexpr int multi_vr() {
-    return sizeof(typename Traits<ST>::T) == 8 ? 56 : 48;
+    return multi_rows((int)sizeof(typename Traits<ST>::T), NS);
 }
 template <int ST>
 constexpr int max_steps_per_pass() {
-    return (64 - multi_vr<ST>()) / 2;
+    return kMaxPassSteps;
 }
 
 // Row y+1 / y-1 of a value held by every lane of a wavefront whose lane i holds row
@@ -87,7 +90,7 @@ template <int MODEL, int ST, int NS>
 __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
     using T = typename Traits<ST>::T;
     using R = typename Traits<ST>::R;
-    constexpr int VR = multi_vr<ST>();
+    constexpr int VR = multi_vr<ST, NS>();
     static_assert(NS >= 1 && VR + 2 * NS <= 64, "halo rows do not fit in a wavefront");
     const int lane = threadIdx.x & 63;
     const int nx = a.nx, ny = a.ny;

@@ -54,12 +54,22 @@ __device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const 
     } else {
         const R om = sizeof(R) == 8 ? R(a.omega) : R(a.omega_f);
         const R lo = sizeof(R) == 8 ? R(a.lambda_odd) : R(a.lambda_odd_f);
-        R d0, jx0, jy0;
-        if constexpr (sizeof(R) == 4) moments_shifted(f, d0, jx0, jy0);
-        if constexpr (MODEL == PLBM_BGK || MODEL == PLBM_BGK_SPLIT) collide_bgk_shifted<R>(f, om);
-        else if constexpr (MODEL == PLBM_TRT) collide_trt_shifted<R>(f, om, lo);
-        else collide_rr_shifted<R>(f, om);
-        if constexpr (sizeof(R) == 4) restore_conservation(f, d0, jx0, jy0);
+        if constexpr (sizeof(R) == 4 && (MODEL == PLBM_BGK || MODEL == PLBM_BGK_SPLIT)) {
+            collide_bgk_shifted_conserving<R>(f, om);
+        } else if constexpr (sizeof(R) == 4 && MODEL == PLBM_TRT) {
+            collide_trt_shifted_conserving<R>(f, om, lo);
+        } else if constexpr (sizeof(R) == 4) {
+            R d0, jx0, jy0;
+            moments_shifted(f, d0, jx0, jy0);
+            collide_rr_shifted<R>(f, om);
+            restore_conservation(f, d0, jx0, jy0);
+        } else if constexpr (MODEL == PLBM_BGK || MODEL == PLBM_BGK_SPLIT) {
+            collide_bgk_shifted<R>(f, om);
+        } else if constexpr (MODEL == PLBM_TRT) {
+            collide_trt_shifted<R>(f, om, lo);
+        } else {
+            collide_rr_shifted<R>(f, om);
+        }
     }
 }
 

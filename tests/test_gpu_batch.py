@@ -79,6 +79,7 @@ def test_batch_autotune_and_reductions(P, gpu):
 
 
 @pytest.mark.parametrize("storage,model,ns", [("f64", "trt", 4), ("f64", "rr", 3), ("f64", "bgk_split", 2),
+                                              ("f64", "trt", 6), ("f64", "bgk", 8),
                                               ("f32s", "bgk", 8), ("f16s", "trt", 6)])
 @pytest.mark.parametrize("nsteps", [7, 30])
 def test_batch_passes(P, gpu, storage, model, ns, nsteps):

@@ -1,4 +1,4 @@
-"""Temporal blocking of plbm_run (passes of 2..4 fused steps per launch, intermediate
+"""Temporal blocking of plbm_run (passes of 2..8 fused steps per launch, intermediate
 states in registers): in FP64 storage bit-identical to the single-step kernel (and the
 oracle) for every model, pass depth and grid shape (ragged strips and wavefronts, grids
 smaller than one wavefront), with the reference's slot contents after the run (latest
@@ -45,7 +45,7 @@ def test_passes_equal_single_steps_fp64(P, gpu, oracle, model, nx, ny):
     ref, ri, rn = run(P, "f64", model, 1, nx, ny, nsteps)
     want_new, want_prev = ref.download_f(ri), ref.download_f(rn)
     assert (ri, rn) == (1, 0)  # 11 steps from (0, 1): the reference's parity
-    for ns in (2, 3, 4):
+    for ns in range(2, 9):
         lat, i, n = run(P, "f64", model, ns, nx, ny, nsteps)
         assert (i, n) == (ri, rn)
         np.testing.assert_array_equal(lat.download_f(i), want_new)
@@ -70,7 +70,7 @@ def test_passes_match_single_steps_shifted(P, gpu, storage, model):
     nx, ny, nsteps = 150, 100, 10
     ref, ri, rn = run(P, storage, model, 1, nx, ny, nsteps)
     r0, u0, v0 = ref.macros(ri)
-    for ns in ((2, 3, 4) if storage == "f64s" else (2, 3, 4, 5, 6, 7, 8)):
+    for ns in range(2, 9):
         lat, i, n = run(P, storage, model, ns, nx, ny, nsteps)
         assert (i, n) == (ri, rn)
         r, u, v = lat.macros(i)
@@ -163,9 +163,10 @@ def test_autotune_picks_a_depth(P, gpu):
     lat.set_params(1.9, 0.25)
     lat.equilibrium_init(0, *fields(n, n, 2))
     ms = lat.autotune("trt", 0, reps=3)
-    assert ms > 0 and lat.get_steps_per_launch("trt") in (1, 2, 3, 4)
+    assert ms > 0 and lat.get_steps_per_launch("trt") in range(1, 9)
+    lat.set_steps_per_launch("trt", 8)
     with pytest.raises(P.PlbmError):
-        lat.set_steps_per_launch("trt", 5)  # FP64 passes hold at most 4 steps
+        lat.set_steps_per_launch("trt", 9)  # passes hold at most 8 steps
     lat32 = P.Lattice(64, 64, 2, "f32s")
     lat32.set_steps_per_launch("trt", 8)
     with pytest.raises(P.PlbmError):
@@ -205,7 +206,7 @@ def test_pass_and_run_errors(P, gpu):
     with pytest.raises(P.PlbmError, match="distinct"):
         lat.run_pass("trt", 2, 1, 1)
     with pytest.raises(P.PlbmError, match="steps per pass"):
-        lat.run_pass("trt", 5, 0, 1)
+        lat.run_pass("trt", 9, 0, 1)
     with pytest.raises(P.PlbmError, match="unknown collision"):
         lat.run_pass(7, 2, 0, 1)
     with pytest.raises(P.PlbmError, match="unknown collision"):
@@ -216,7 +217,7 @@ def test_pass_and_run_errors(P, gpu):
         lat.step_op(9, "trt", 0, 1, 1.0)
     b = P.Batch([(32, 32)], "f64", "bgk")
     with pytest.raises(P.PlbmError, match="steps per pass"):
-        b.set_steps_per_pass(5)
+        b.set_steps_per_pass(9)
     # a valid call after the errors still works and equals single steps
     lat.equilibrium_init(0, *fields(64, 64, 3))
     lat.run_pass("trt", 2, 0, 1)
