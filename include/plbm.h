@@ -237,17 +237,16 @@ double plbm_magic_number(double le, double ld);
 int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap);
 int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap);
 /* Temporal blocking of plbm_run: steps advanced per pass over HBM (1 = the single-step
- * kernel; 2..4 for F64 / F64S storage or 2..8 for F32S / F16S = that many fused steps
- * per launch, each wavefront walking a segment of columns with the intermediate states
- * in registers; 0 = chosen from the grid size: 4 when the grid gives at least 2048
- * wavefronts, else 1).  In F64 storage the results are bit-identical for every setting;
+ * kernel; 2..8 = that many fused steps per launch, each wavefront walking a segment of
+ * columns with the intermediate states in registers; 0 = chosen from the grid size: 8
+ * when the grid gives at least 2048 wavefronts of 48 x 64 cells, else 1).  In F64 storage the results are bit-identical for every setting;
  * the shifted storages agree to roundoff (their collisions allow FMA contraction, which
  * the compiler places per kernel).  plbm_run keeps the reference's slot contents (latest
  * in *iold, the state one step earlier in *inew); with passes of more than one step it
  * may exchange the device storage of the two slots, so addresses from plbm_buffer_ptr
  * must be fetched again after it.  The get variant reports the effective value. */
 int plbm_set_steps_per_launch(plbm_lattice *h, int model, int steps);
-/* One pass of `steps` (1..4, or 1..8 for F32S / F16S) fused steps from buffer src to
+/* One pass of `steps` (1..8) fused steps from buffer src to
  * buffer dst, without slot bookkeeping and without writing the intermediate states:
  * the building block of plbm_run.  Asynchronous. */
 int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst);
@@ -260,7 +259,7 @@ int plbm_set_pass_segments(plbm_lattice *h, int segments);
 int plbm_get_pass_segments(const plbm_lattice *h, int *segments);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
  * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
- * of 2..4 (2..8) steps against it per lattice update; keep the fastest for this lattice and
+ * of 2..8 steps against it per lattice update; keep the fastest for this lattice and
  * cache it per (device, storage, model, grid) for the process.  *best_ms (may be NULL)
  * receives the winner's time per lattice-update step. */
 int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms);
@@ -300,7 +299,7 @@ int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int
 int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
                           int *resident_block);
 /* Steps per launch of the tiled (non-resident) cases: 1 = one step per launch (tile
- * table); 2..4 (FP64 / F64S) or 2..8 (F32S / F16S) = temporally blocked passes, as
+ * table); 2..8 = temporally blocked passes, as
  * plbm_set_steps_per_launch for one lattice (bit-identical in F64 storage).  Default 1. */
 int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps);
 int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps);

@@ -141,8 +141,8 @@ class Batch:
         return tuple(x.value for x in v)
 
     def set_steps_per_pass(self, steps: int):
-        """Steps per launch of the tiled cases (1: one step per launch; 2..4 FP64, 2..8
-        FP32 / FP16 storage: temporally blocked passes)."""
+        """Steps per launch of the tiled cases (1: one step per launch; 2..8: temporally
+        blocked passes)."""
         check(lib().plbm_batch_set_steps_per_pass(self.h, steps))
 
     def get_steps_per_pass(self) -> int:

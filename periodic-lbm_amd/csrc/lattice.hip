@@ -412,7 +412,7 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
         if (status != PLBM_OK) break;
     }
     // steps per pass of plbm_run: the single-step kernel at the shape just chosen against
-    // passes of 2..4 steps, compared per lattice update
+    // passes of 2..8 steps, compared per lattice update
     best.ns = 1;
     for (int ns = 2; ns <= max_steps_per_pass(h->storage) && status == PLBM_OK; ++ns) {
         double tmin = 1e300;

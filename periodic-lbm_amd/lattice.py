@@ -94,9 +94,8 @@ class Lattice:
         return b.value, r.value
 
     def set_steps_per_launch(self, model: str | int, steps: int):
-        """Temporal blocking depth of run(): 1 single steps, 2..4 (FP64 storage) or 2..8
-        (FP32 / FP16 storage) fused steps per pass, 0 automatic.  FP64 results are
-        bit-identical for every value."""
+        """Temporal blocking depth of run(): 1 single steps, 2..8 fused steps per pass,
+        0 automatic.  FP64 results are bit-identical for every value."""
         check(lib().plbm_set_steps_per_launch(self.h, _abi.MODEL_IDS.get(model, model), steps))
 
     def run_pass(self, model: str | int, steps: int, src: int, dst: int):

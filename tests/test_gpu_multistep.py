@@ -97,7 +97,7 @@ def test_step_counts_and_slots(P, gpu, nsteps):
 
 
 def test_full_size_passes_bit_exact_4096(P, gpu, oracle):
-    """4096^2 TRT, 9 steps in passes of 4 (the automatic choice at this size) against
+    """4096^2 TRT, 9 steps in passes of 8 (the automatic choice at this size) against
     the oracle."""
     n, nsteps = 4096, 9
     print(f"case multistep 4096 trt {nsteps}", flush=True)
@@ -105,7 +105,7 @@ def test_full_size_passes_bit_exact_4096(P, gpu, oracle):
     lat = P.Lattice(n, n, 2, "f64")
     omega = 1.0 / (3 * 0.01 + 0.5)
     lat.set_params(omega, 0.25)
-    assert lat.get_steps_per_launch("trt") == 4
+    assert lat.get_steps_per_launch("trt") == 8
     lat.equilibrium_init(0, rho, ux, uy)
     i, n_ = lat.run("trt", nsteps, 0, 1)
     fa, fb = po.new_f(n, n), po.new_f(n, n)
