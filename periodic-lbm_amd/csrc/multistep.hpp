@@ -75,17 +75,28 @@ __device__ __forceinline__ void wait_vm_at_most(int n) {
     else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1
+// Value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1.  For 8-byte
+// values DPP lane shifts (VALU moves): the FP64 pass is bound by instruction issue and
+// latency, and ds_bpermute shuffles put an LDS round trip between consecutive levels
+// (8192^2 TRT FP64, 8 steps per pass: 2.88 ms with ds_bpermute, 2.58 ms with DPP).  For
+// 4-byte values the shuffles measured at least as fast (8192^2 TRT, ms per step with
+// shuffles / DPP: FP32 storage 0.175 / 0.178, FP16 storage 0.227 / 0.255).
 template <class R>
 __device__ __forceinline__ R pull_y(int q, R v) {
-    if (cy(q) > 0) return __shfl_up(v, 1, 64);
-    if (cy(q) < 0) return __shfl_down(v, 1, 64);
-    return v;
+    if constexpr (sizeof(R) == 8) {
+        if (cy(q) > 0) return row_minus(v);
+        if (cy(q) < 0) return row_plus(v);
+        return v;
+    } else {
+        if (cy(q) > 0) return __shfl_up(v, 1, 64);
+        if (cy(q) < 0) return __shfl_down(v, 1, 64);
+        return v;
+    }
 }
 
 // The calling wavefront advances rows [VR*wy, VR*wy + VR) (clipped to the grid) x columns
 // [x0, x0 + ncols) (0 <= x0, x0 + ncols <= nx) of the grid described by `a` NS steps,
-// a.src -> a.dst.  All 64 lanes must call it together (lane shuffles).
+// a.src -> a.dst.  All 64 lanes must call it together (lane shifts and shuffles).
 template <int MODEL, int ST, int NS>
 __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
     using T = typename Traits<ST>::T;
