@@ -351,8 +351,7 @@ __device__ inline void collide_rr_shifted(R g[9], R omega) {
 // without changing the operator: BGK takes geq_1, geq_2 from the momentum balances and
 // geq_0 from the mass balance of the equilibrium; TRT takes the antisymmetric parts of
 // pairs (1,3), (2,4) from those of (5,7), (8,6) and the rest population from the
-// symmetric parts.  What remains is the unbiased rounding of each update, and the
-// measure-and-correct pass (restore_conservation below) is not needed.
+// symmetric parts.  What remains is the unbiased rounding of each update.
 template <class R>
 __device__ inline void collide_bgk_shifted_conserving(R g[9], R omega) {
 #pragma clang fp contract(fast)
@@ -397,33 +396,18 @@ __device__ inline void collide_trt_shifted_conserving(R g[9], R le, R lo) {
     g[5] = o5; g[6] = o6; g[7] = o7; g[8] = o8;
 }
 
-// Conservation restoration for reduced-precision arithmetic (the RR operator in FP32).  In FP32 the weights
-// w_q (4/9, 1/9, 1/36) and the TRT/RR prefactors are not representable, so the
-// equilibrium's zeroth and first moments miss rho and j by a fixed relative amount
-// (~1e-8) in every cell and step; over thousands of steps that bias accumulates in the
-// velocity field.  Subtracting w_q*(dd + 3 c_q.dj) from the post-collision populations
-// (dd, dj = moment change the collision produced) returns mass and momentum to their
-// pre-collision values up to unbiased rounding.  In exact arithmetic the correction is
-// zero for every model (each conserves mass and momentum), so it changes no result
-// beyond rounding.
+// Conservation closure for the RR operator in FP32 (the same idea as above): RR conserves
+// rho and j exactly in exact arithmetic, so after the update g_1, g_2 are taken from the
+// momentum balances and g_0 from the mass balance against the pre-collision moments
+// (d0, jx0, jy0); the operator's own formulas for those three populations are then dead
+// code.
 template <class R>
-__device__ inline void restore_conservation(R g[9], R d0, R jx0, R jy0) {
+__device__ inline void close_balances(R g[9], R d0, R jx0, R jy0) {
 #pragma clang fp contract(fast)
-    R d1, jx1, jy1;
-    moments_shifted(g, d1, jx1, jy1);
-    const R dd = d1 - d0, dx = jx1 - jx0, dy = jy1 - jy0;
-    const R a0 = R(W0) * dd, as = R(WS) * dd, ad = R(WD) * dd;
-    const R bsx = R(3.0 * WS) * dx, bsy = R(3.0 * WS) * dy;
-    const R bdp = R(3.0 * WD) * (dx + dy), bdm = R(3.0 * WD) * (dx - dy);
-    g[0] -= a0;
-    g[1] -= as + bsx;
-    g[3] -= as - bsx;
-    g[2] -= as + bsy;
-    g[4] -= as - bsy;
-    g[5] -= ad + bdp;
-    g[7] -= ad - bdp;
-    g[8] -= ad + bdm;
-    g[6] -= ad - bdm;
+    const R e57 = g[5] - g[7];
+    g[1] = g[3] + (jx0 - (e57 + (g[8] - g[6])));
+    g[2] = g[4] + (jy0 - (e57 + (g[6] - g[8])));
+    g[0] = d0 - (((g[5] + g[7]) + (g[6] + g[8])) + ((g[1] + g[3]) + (g[2] + g[4])));
 }
 
 }  // namespace plbm

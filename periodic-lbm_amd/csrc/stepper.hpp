@@ -62,7 +62,7 @@ __device__ __forceinline__ void collide_cell(typename Traits<ST>::R f[9], const 
             R d0, jx0, jy0;
             moments_shifted(f, d0, jx0, jy0);
             collide_rr_shifted<R>(f, om);
-            restore_conservation(f, d0, jx0, jy0);
+            close_balances(f, d0, jx0, jy0);
         } else if constexpr (MODEL == PLBM_BGK || MODEL == PLBM_BGK_SPLIT) {
             collide_bgk_shifted<R>(f, om);
         } else if constexpr (MODEL == PLBM_TRT) {
