@@ -246,11 +246,9 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
             if (k < NS) {
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
-                    R v = f[q];
-                    if constexpr (ST == PLBM_F16S) v = R(T(v * a.scale)) * a.inv_scale;
                     W[k][2][q] = W[k][1][q];
                     W[k][1][q] = W[k][0][q];
-                    W[k][0][q] = pull_y(q, v);
+                    W[k][0][q] = pull_y(q, f[q]);
                 }
             } else if (out) {
                 const size_t co = (size_t)(x0 - 2 * NS + i) * cs;

@@ -59,8 +59,9 @@ def test_passes_equal_single_steps_fp64(P, gpu, oracle, model, nx, ny):
     np.testing.assert_array_equal(want_new[:ny], latest[:ny])
 
 
-# velocity tolerance after 10 steps: FP64 roundoff, FP32 roundoff, one FP16 storage
-# rounding (scale 2^10) of an intermediate state
+# velocity tolerance after 10 steps: FP64 roundoff, FP32 roundoff (FMA placement), and
+# for FP16 storage the FP16 roundings (scale 2^10) that single steps make of the states
+# a pass keeps in FP32 registers
 SHIFTED_TOL = {"f64s": 1e-14, "f32s": 1e-6, "f16s": 2e-4}
 
 
