@@ -124,8 +124,9 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
         if (c < 0) c += nx;
         return (size_t)c * cs;
     };
-    // the next column's loads are issued one iteration ahead, so that their latency
-    // overlaps the collisions of the current column
+    // The next columns' loads are issued ahead (two columns into LDS for 8-byte storage,
+    // one column into registers otherwise), so that their latency overlaps the
+    // collisions of the current column.
     constexpr bool kStage = sizeof(T) == 8;
     T nxt[9];
     // Staged path.  When the wavefront's first row and ny are even, no pair of rows
