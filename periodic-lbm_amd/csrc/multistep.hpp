@@ -146,7 +146,9 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
             int yy = (VR * wy - NS + m) % ny;
             return yy < 0 ? yy + ny : yy;
         };
-        pairs = (NS % 2 == 0) && (ny % 2 == 0);
+        // (and the planes start on 16-byte boundaries: ldd even, buffer 16-byte aligned)
+        pairs = (NS % 2 == 0) && (ny % 2 == 0) && (L % 2 == 0) &&
+                (reinterpret_cast<uintptr_t>(s) % 16 == 0);
         if (pairs) {
             const unsigned off = (unsigned)row_of(2 * (lane & 31)) * 8u;
             stage_a = off + (lane >> 5) * (unsigned)(L * 8);  // populations q, q + 1
