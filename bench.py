@@ -300,6 +300,7 @@ def batch_leg(P, torch, args):
     out["tune_ms_per_step"] = b.autotune(10)
     out["launch"] = dict(zip(("ty", "tx", "resident_cells", "resident_block"), b.get_launch()))
     out["launch"]["steps_per_pass"] = b.get_steps_per_pass()
+    out["launch"]["pass_columns"] = b.get_pass_columns()
     b.run(20)
     b.sync()
     t0 = time.perf_counter()

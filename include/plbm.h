@@ -303,6 +303,11 @@ int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_c
  * plbm_set_steps_per_launch for one lattice (bit-identical in F64 storage).  Default 1. */
 int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps);
 int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps);
+/* Columns each wavefront of a pass walks (engine extension, no counterpart in the
+ * reference): a case is cut into segments of this many columns.  Default 64; the
+ * autotuner tries 64, 128 and 256.  Results do not depend on it. */
+int plbm_batch_set_pass_columns(plbm_batch *b, int columns);
+int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns);
 /* Autotuner: time `steps` (even) steps of every plan candidate (tile shapes, steps per
  * pass, resident threshold) on a scratch copy of the arena and keep the fastest;
  * *best_ms (may be NULL) = its time per step. */

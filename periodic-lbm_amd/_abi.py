@@ -98,6 +98,8 @@ SIGNATURES = {
     "plbm_batch_get_launch": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "plbm_batch_set_steps_per_pass": (_i, [_vp, _i]),
     "plbm_batch_get_steps_per_pass": (_i, [_vp, _ip]),
+    "plbm_batch_set_pass_columns": (_i, [_vp, _i]),
+    "plbm_batch_get_pass_columns": (_i, [_vp, _ip]),
     "plbm_batch_autotune": (_i, [_vp, _i, _dp]),
     "plbm_adjoint_create": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "plbm_adjoint_destroy": (_i, [_vp]),

@@ -150,6 +150,16 @@ class Batch:
         check(lib().plbm_batch_get_steps_per_pass(self.h, ctypes.byref(v)))
         return v.value
 
+    def set_pass_columns(self, columns: int):
+        """Columns each wavefront of a pass walks (segments of a case); results do not
+        depend on it."""
+        check(lib().plbm_batch_set_pass_columns(self.h, columns))
+
+    def get_pass_columns(self) -> int:
+        v = ctypes.c_int()
+        check(lib().plbm_batch_get_pass_columns(self.h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, steps=10) -> float:
         ms = ctypes.c_double()
         check(lib().plbm_batch_autotune(self.h, steps, ctypes.byref(ms)))

@@ -19,8 +19,8 @@
 //    wavefront per 56 x 64 (FP64, up to 4 steps) or 48 x 64 block of a case, listed in a
 //    wave table in place of the tile table.
 //
-// Tile shape (ty, tx), steps per pass and the resident threshold are picked by the
-// project's autotuner (plbm_batch_autotune), which times candidates on a scratch copy of
+// Tile shape (ty, tx), steps per pass, columns per pass segment and the resident threshold
+// are picked by the project's autotuner (plbm_batch_autotune), which times candidates on a scratch copy of
 // the arena.
 #include <algorithm>
 #include <cmath>
@@ -45,7 +45,7 @@ struct BatchCase {
 };
 
 struct BatchTile {
-    int c, x0, y0, pad;
+    int c, x0, y0, n;  // case, first column, first row (row band with passes), columns
 };
 
 struct BatchCfg {
@@ -53,6 +53,7 @@ struct BatchCfg {
     int resident_cells = 0;    // cases with nx*ny <= this run in k_batch_resident
     int resident_block = 256;
     int ns = 1;                // steps per pass of the tiled cases (> 1: k_batch_multi)
+    int xs = kMultiXS;         // columns per wavefront segment of a pass
 };
 
 template <int ST>
@@ -135,8 +136,7 @@ __global__ void __launch_bounds__(256)
     StepArgs a = case_args<ST>(c, scale, inv_scale);
     a.src = parity ? base + c.elems : base;
     a.dst = parity ? base : base + c.elems;
-    const int x0 = w.x0 * kMultiXS;
-    multi_wave<MODEL, ST, NS>(a, w.y0, x0, min(kMultiXS, c.nx - x0));
+    multi_wave<MODEL, ST, NS>(a, w.y0, w.x0, w.n);
 }
 
 // nsteps steps of one small case per workgroup; only workgroup barriers between steps.
@@ -306,10 +306,11 @@ static int build_plan(plbm_batch *b, const BatchCfg &cfg, BatchTile **d_tiles, i
             res.push_back(c);
             continue;
         }
-        if (cfg.ns > 1) {  // wave table: {case, strip, row block}
+        if (cfg.ns > 1) {  // wave table: {case, first column, row band, columns}
             const int vr = multi_rows(storage_bytes(b->storage), cfg.ns);
-            for (int xs = 0; xs * kMultiXS < k.nx; ++xs)
-                for (int wy = 0; wy * vr < k.ny; ++wy) tiles.push_back({c, xs, wy, 0});
+            for (int x0 = 0; x0 < k.nx; x0 += cfg.xs)
+                for (int wy = 0; wy * vr < k.ny; ++wy)
+                    tiles.push_back({c, x0, wy, std::min(cfg.xs, k.nx - x0)});
         } else {
             for (int x0 = 0; x0 < k.nx; x0 += cfg.tx)
                 for (int y0 = 0; y0 < k.ny; y0 += cfg.ty) tiles.push_back({c, x0, y0, 0});
@@ -728,6 +729,21 @@ int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps) {
     return PLBM_OK;
 }
 
+int plbm_batch_set_pass_columns(plbm_batch *b, int columns) {
+    PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
+    PLBM_REQUIRE(columns >= 1, PLBM_ERR_ARG, "columns per pass segment must be >= 1");
+    PLBM_HIP(hipStreamSynchronize(b->stream));
+    BatchCfg c = b->cfg;
+    c.xs = columns;
+    return set_cfg(b, c);
+}
+
+int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns) {
+    PLBM_REQUIRE(b && columns, PLBM_ERR_ARG, "null argument");
+    *columns = b->cfg.xs;
+    return PLBM_OK;
+}
+
 int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
                           int *resident_block) {
     PLBM_REQUIRE(b && ty && tx && resident_cells && resident_block, PLBM_ERR_ARG, "null argument");
@@ -808,15 +824,17 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
             }
     const int max_ns = max_steps_per_pass(b->storage);
     for (int ns = 2; ns <= max_ns; ns += (ns < 4 ? 1 : 2))
-        for (int res : {0, 4096, 16384, 36864})
-            for (int rb : {256, 1024}) {
-                if (res == 0 && rb != 256) continue;
-                BatchCfg c;
-                c.ns = ns;
-                c.resident_cells = res;
-                c.resident_block = rb;
-                cands.push_back(c);
-            }
+        for (int xs : {kMultiXS, 2 * kMultiXS, 4 * kMultiXS})
+            for (int res : {0, 4096, 16384, 36864})
+                for (int rb : {256, 1024}) {
+                    if (res == 0 && rb != 256) continue;
+                    BatchCfg c;
+                    c.ns = ns;
+                    c.xs = xs;
+                    c.resident_cells = res;
+                    c.resident_block = rb;
+                    cands.push_back(c);
+                }
     BatchCfg best = b->cfg;
     double tbest = 1e300;
     int status = PLBM_OK;

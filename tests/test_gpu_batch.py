@@ -78,20 +78,23 @@ def test_batch_autotune_and_reductions(P, gpu):
         b.set_launch(48, 1, 0, 256)
 
 
-@pytest.mark.parametrize("storage,model,ns", [("f64", "trt", 4), ("f64", "rr", 3), ("f64", "bgk_split", 2),
-                                              ("f64", "trt", 6), ("f64", "bgk", 8),
-                                              ("f32s", "bgk", 8), ("f16s", "trt", 6)])
+@pytest.mark.parametrize("storage,model,ns,xs", [("f64", "trt", 4, 64), ("f64", "rr", 3, 128),
+                                                 ("f64", "bgk_split", 2, 64), ("f64", "trt", 6, 256),
+                                                 ("f64", "bgk", 8, 37), ("f32s", "bgk", 8, 128),
+                                                 ("f16s", "trt", 6, 64)])
 @pytest.mark.parametrize("nsteps", [7, 30])
-def test_batch_passes(P, gpu, storage, model, ns, nsteps):
+def test_batch_passes(P, gpu, storage, model, ns, xs, nsteps):
     """Temporally blocked passes for the tiled cases: FP64 cases equal the single lattice
     bit for bit, shifted storages to roundoff; the resident cases keep stepping one step
     at a time in parallel, and the two agree on the buffer holding the latest state."""
     cases = P.sweep_cases(12, seed=13, sizes=(16, 48, 64, 96, 130))
-    print(f"case batch passes {storage} {model} ns={ns} n={nsteps}", flush=True)
+    print(f"case batch passes {storage} {model} ns={ns} xs={xs} n={nsteps}", flush=True)
     b = P.Batch.from_cases(cases, storage, model)
     b.set_launch(128, 1, 2400, 256)  # cases up to 48^2 resident, the rest in passes
     b.set_steps_per_pass(ns)
     assert b.get_steps_per_pass() == ns
+    b.set_pass_columns(xs)
+    assert b.get_pass_columns() == xs
     b.run(nsteps)
     b.sync()
     tol = {"f64": 0.0, "f32s": 1e-6, "f16s": 2e-4}[storage]
