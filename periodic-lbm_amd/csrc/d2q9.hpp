@@ -251,6 +251,18 @@ __device__ inline void moments_shifted(const R g[9], R &drho, R &jx, R &jy) {
     jy = ((g[5] - g[7]) + (g[6] - g[8])) + (g[2] - g[4]);
 }
 
+// 1/x for the shifted forms: in FP32 the hardware reciprocal (v_rcp_f32, 1 ulp) instead of
+// the correctly rounded division sequence (about ten instructions per cell in an
+// instruction-bound pass); the FP32 forms are held to error bounds against FP64, not to
+// bit equality.  FP64 divides exactly.
+template <class R>
+__device__ __forceinline__ R recip(R x) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_amdgcn_rcpf(x);
+    else
+        return R(1) / x;
+}
+
 template <class R>
 __device__ inline void collide_bgk_shifted(R g[9], R omega) {
 #pragma clang fp contract(fast)
@@ -301,7 +313,7 @@ __device__ inline void collide_rr_shifted(R g[9], R omega) {
     R drho, jx, jy;
     moments_shifted(g, drho, jx, jy);
     const R rho = R(1) + drho;
-    const R inv = R(1) / rho;
+    const R inv = recip(rho);
     const R ux = jx * inv, uy = jy * inv;
     const R uxx = ux * ux, uyy = uy * uy, uxxy = uxx * uy, uyyx = uyy * ux, uxxyy = uxx * uyy;
     const R p0 = R(-1.5) * (uxx + uyy);
@@ -357,7 +369,7 @@ __device__ inline void collide_bgk_shifted_conserving(R g[9], R omega) {
 #pragma clang fp contract(fast)
     R drho, jx, jy;
     moments_shifted(g, drho, jx, jy);
-    const R inv = R(1) / (R(1) + drho);
+    const R inv = recip(R(1) + drho);
     R geq[9];
     equilibrium_shifted(drho, jx * inv, jy * inv, geq);
     const R e57 = geq[5] - geq[7];
