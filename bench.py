@@ -477,6 +477,10 @@ def main():
         "measured_copy_gbs": round(copy_gbs, 1),
         "frac_of_measured_copy": round(achieved / copy_gbs, 4),
         "single_step_frac_of_measured_copy": round(f64["single_step_gbs"] / copy_gbs, 4),
+        # the baseline's own "% roofline": MLUPS x one step's 144 B per lattice update over
+        # the measured copy bandwidth (above 1 once several steps share one sweep)
+        "lu_bytes_rate_over_measured_copy": round(
+            f64["mlups"] * 1e6 * BYTES_PER_LU["f64"] / 1e9 / copy_gbs, 3),
         "storage_forms": {
             k: {kk: (round(vv, 6) if isinstance(vv, float) and vv > 1e-3 else vv)
                 for kk, vv in v.items()} for k, v in results.items()},
