@@ -80,7 +80,8 @@ def test_batch_autotune_and_reductions(P, gpu):
 
 @pytest.mark.parametrize("storage,model,ns,xs", [("f64", "trt", 4, 64), ("f64", "rr", 3, 128),
                                                  ("f64", "bgk_split", 2, 64), ("f64", "trt", 6, 256),
-                                                 ("f64", "bgk", 8, 37), ("f32s", "bgk", 8, 128),
+                                                 ("f64", "bgk", 8, 37), ("f64", "trt", 5, 1),
+                                                 ("f32s", "bgk", 8, 128),
                                                  ("f16s", "trt", 6, 64)])
 @pytest.mark.parametrize("nsteps", [7, 30])
 def test_batch_passes(P, gpu, storage, model, ns, xs, nsteps):
