@@ -75,6 +75,105 @@ __device__ __forceinline__ void wait_vm_at_most(int n) {
     else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// The next columns of a wavefront's walk along x, staged in LDS (8-byte values).  The
+// wavefront's lane j holds row first_row + j (mod ny) of every population; column c is
+// staged in buffer c % kStageBufs, population q in slot q (64 values).  When the first
+// row and ny are even (and the planes 16-byte aligned), no pair of rows (2m, 2m+1)
+// straddles the periodic wrap and one 16-byte load per lane moves two rows: lanes 0-31
+// fill the slot of population q, lanes 32-63 that of q + 1 (5 loads per column; the fifth
+// moves population 8 twice, the second copy into the padding slot).  Otherwise lane j
+// moves the 4-byte half (j & 1) of row (j >> 1) and then of row 32 + (j >> 1) of each
+// population (18 loads per column).
+struct ColumnStage {
+    double *buf0;      // this wavefront's kStageBufs buffers of kStageSlots x 64 values
+    unsigned off_a, off_b;
+    bool pairs;
+    int loads;         // vector memory operations per staged column
+
+    __device__ __forceinline__ void init(double *mem, const double *s, size_t L, int ny,
+                                         int first_row, int lane) {
+        buf0 = mem;
+        auto row_of = [&](int m) {
+            int yy = (first_row + m) % ny;
+            return yy < 0 ? yy + ny : yy;
+        };
+        pairs = (first_row % 2 == 0) && (ny % 2 == 0) && (L % 2 == 0) &&
+                (reinterpret_cast<uintptr_t>(s) % 16 == 0);
+        if (pairs) {
+            const unsigned off = (unsigned)row_of(2 * (lane & 31)) * 8u;
+            off_a = off + (lane >> 5) * (unsigned)(L * 8);  // populations q, q + 1
+            off_b = off;                                     // population 8 twice
+        } else {
+            off_a = (unsigned)row_of(lane >> 1) * 8u + (lane & 1) * 4u;
+            off_b = (unsigned)row_of(32 + (lane >> 1)) * 8u + (lane & 1) * 4u;
+        }
+        loads = pairs ? 5 : 18;
+    }
+
+    // Issue the loads of the column whose planes start at s + cc (plane stride L) into
+    // buffer buf.
+    __device__ __forceinline__ void fetch(const double *s, size_t cc, size_t L, int buf) {
+        double *slots = buf0 + buf * kStageSlots * 64;
+        if (pairs) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const char *plane = reinterpret_cast<const char *>(s + cc + 2 * m * L);
+                __builtin_amdgcn_global_load_lds((gbl_void *)(plane + off_a),
+                                                 (lds_void *)(slots + 2 * m * 64), 16, 0, 0);
+            }
+            const char *plane = reinterpret_cast<const char *>(s + cc + 8 * L);
+            __builtin_amdgcn_global_load_lds((gbl_void *)(plane + off_b),
+                                             (lds_void *)(slots + 8 * 64), 16, 0, 0);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const char *plane = reinterpret_cast<const char *>(s + cc + q * L);
+                __builtin_amdgcn_global_load_lds((gbl_void *)(plane + off_a),
+                                                 (lds_void *)(slots + q * 64), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((gbl_void *)(plane + off_b),
+                                                 (lds_void *)(slots + q * 64 + 32), 4, 0, 0);
+            }
+        }
+        // keep the stores of this iteration behind these loads in issue order (wait()
+        // counts on it)
+        asm volatile("" ::: "memory");
+    }
+
+    // Wait until column i of a walk of `iters` columns is in LDS.  The prologue issued
+    // the loads of columns 0 .. kStageAhead-1; iteration t issues those of column
+    // t + kStageAhead (if it exists) and then 9 stores (if t >= first_store).  Vector
+    // memory operations complete in issue order, so it suffices that no more remain
+    // outstanding than were issued after the loads of column i.  Also waits for the LDS
+    // reads of the previous column, whose buffer the next loads overwrite.
+    __device__ __forceinline__ void wait(int i, int iters, int first_store) const {
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        int after = 0, t0;
+        if (i < kStageAhead) {
+            for (int j = i + 1; j < kStageAhead; ++j) after += j < iters ? loads : 0;
+            t0 = 0;
+        } else {
+            after += i - kStageAhead >= first_store ? 9 : 0;
+            t0 = i - kStageAhead + 1;
+        }
+        for (int t = t0; t < i; ++t) {
+            after += t + kStageAhead < iters ? loads : 0;
+            after += t >= first_store ? 9 : 0;
+        }
+        wait_vm_at_most(after);
+    }
+
+    __device__ __forceinline__ const double *column(int i) const {
+        return buf0 + (i % kStageBufs) * kStageSlots * 64;
+    }
+};
+
+// This wavefront's LDS staging buffers (four wavefronts per workgroup).
+__device__ __forceinline__ double *stage_buffers() {
+    __shared__ double stage_mem[kStageWaves][kStageBufs][kStageSlots][64];
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & (kStageWaves - 1);
+    return &stage_mem[wib][0][0][0];
+}
+
 // Value at row y - cy_q: cy = +1 comes from lane - 1, cy = -1 from lane + 1.  For 8-byte
 // values DPP lane shifts (VALU moves): the FP64 pass is bound by instruction issue and
 // latency, and ds_bpermute shuffles put an LDS round trip between consecutive levels
@@ -129,63 +228,13 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     // collisions of the current column.
     constexpr bool kStage = sizeof(T) == 8;
     T nxt[9];
-    // Staged path.  When the wavefront's first row and ny are even, no pair of rows
-    // (2m, 2m+1) of the wavefront straddles the periodic wrap, and one 16-byte load per
-    // lane moves two rows: lanes 0-31 fill the slot of population q, lanes 32-63 that of
-    // q + 1 (5 loads per column; the fifth moves population 8 twice, the second copy into
-    // the padding slot).  Otherwise lane j moves the 4-byte half (j & 1) of row (j >> 1)
-    // and then of row 32 + (j >> 1) of each population (18 loads per column).
-    unsigned stage_a = 0, stage_b = 0;
-    bool pairs = false;
-    double *stage = nullptr;
-    if constexpr (kStage) {
-        __shared__ double stage_mem[kStageWaves][kStageBufs][kStageSlots][64];
-        const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & (kStageWaves - 1);
-        stage = &stage_mem[wib][0][0][0];
-        auto row_of = [&](int m) {
-            int yy = (VR * wy - NS + m) % ny;
-            return yy < 0 ? yy + ny : yy;
-        };
-        // (and the planes start on 16-byte boundaries: ldd even, buffer 16-byte aligned)
-        pairs = (NS % 2 == 0) && (ny % 2 == 0) && (L % 2 == 0) &&
-                (reinterpret_cast<uintptr_t>(s) % 16 == 0);
-        if (pairs) {
-            const unsigned off = (unsigned)row_of(2 * (lane & 31)) * 8u;
-            stage_a = off + (lane >> 5) * (unsigned)(L * 8);  // populations q, q + 1
-            stage_b = off;                                     // population 8 twice
-        } else {
-            stage_a = (unsigned)row_of(lane >> 1) * 8u + (lane & 1) * 4u;
-            stage_b = (unsigned)row_of(32 + (lane >> 1)) * 8u + (lane & 1) * 4u;
-        }
-    }
-    const int stage_loads = pairs ? 5 : 18;  // vector memory operations per staged column
+    ColumnStage st{};
+    if constexpr (kStage)
+        st.init(stage_buffers(), reinterpret_cast<const double *>(s), L, ny, VR * wy - NS, lane);
     auto fetch = [&](int c, int buf) {
         const size_t cc = column(c);
         if constexpr (kStage) {
-            double *slots = stage + buf * kStageSlots * 64;
-            if (pairs) {
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const char *plane = reinterpret_cast<const char *>(s + cc + 2 * m * L);
-                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_a),
-                                                     (lds_void *)(slots + 2 * m * 64), 16, 0, 0);
-                }
-                const char *plane = reinterpret_cast<const char *>(s + cc + 8 * L);
-                __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_b),
-                                                 (lds_void *)(slots + 8 * 64), 16, 0, 0);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const char *plane = reinterpret_cast<const char *>(s + cc + q * L);
-                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_a),
-                                                     (lds_void *)(slots + q * 64), 4, 0, 0);
-                    __builtin_amdgcn_global_load_lds((gbl_void *)(plane + stage_b),
-                                                     (lds_void *)(slots + q * 64 + 32), 4, 0, 0);
-                }
-            }
-            // keep the stores of this iteration behind these loads in issue order (the
-            // wait below counts on it)
-            asm volatile("" ::: "memory");
+            st.fetch(reinterpret_cast<const double *>(s), cc, L, buf);
         } else {
 #pragma unroll
             for (int q = 0; q < 9; ++q) nxt[q] = s[cc + q * L + y];
@@ -198,27 +247,8 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
         if (j < iters) fetch(x0 - NS + j, j);
     for (int i = 0; i < iters; ++i) {
         if constexpr (kStage) {
-            // The previous column's LDS reads are complete before the staging loads
-            // below overwrite the buffer they read.
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            // Vector memory operations complete in issue order.  The prologue issues the
-            // staging loads of each of columns 0 .. ahead-1; iteration t those of column
-            // t + ahead (if it exists) and then its 9 stores (if t >= 2 NS).  Count what
-            // was issued after the loads of column i.
-            int after = 0, t0;
-            if (i < ahead) {
-                for (int j = i + 1; j < ahead; ++j) after += j < iters ? stage_loads : 0;
-                t0 = 0;
-            } else {
-                after += i - ahead >= 2 * NS ? 9 : 0;
-                t0 = i - ahead + 1;
-            }
-            for (int t = t0; t < i; ++t) {
-                after += t + ahead < iters ? stage_loads : 0;
-                after += t >= 2 * NS ? 9 : 0;
-            }
-            wait_vm_at_most(after);
-            const double *slot = stage + (i % kStageBufs) * kStageSlots * 64;
+            st.wait(i, iters, 2 * NS);
+            const double *slot = st.column(i);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 W[0][2][q] = W[0][1][q];

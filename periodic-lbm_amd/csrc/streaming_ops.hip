@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(256)
 // storage (temporal blocking as the lattice Boltzmann pass kernel): a wavefront owns 64
 // consecutive rows (56 with output, NS halo rows on each side) and walks a strip of
 // kMultiXS columns, keeping for every level k < NS the last three columns of its own row
-// in registers; y-neighbours come from the adjacent lanes (DPP).  When level k receives a
+// in registers; y-neighbours come from the adjacent lanes (DPP).  The columns are staged
+// in LDS two ahead of the walk.  When level k receives a
 // column, level k+1 gets the column before it: the operator's 3x3 stencil per direction,
 // then the collision, exactly as k_stream_fd<OP, F64, MODEL> computes them.
 template <int OP, int MODEL, int NS>
@@ -174,13 +175,23 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int q = 0; q < 9; ++q) W[k][j][q] = 0.0;
     const int iters = ncols + 2 * NS;
+    // the next columns are staged in LDS ahead of the walk (as in the LBM pass kernel),
+    // so their latency overlaps the operator and collisions of the current column
+    ColumnStage st{};
+    st.init(stage_buffers(), s, L, ny, VR * wy - NS, lane);
+#pragma unroll
+    for (int j = 0; j < kStageAhead; ++j)
+        if (j < iters) st.fetch(s, column(x0 - NS + j), L, j % kStageBufs);
     for (int i = 0; i < iters; ++i) {
         double f[9];
+        st.wait(i, iters, 2 * NS);
         {
-            const size_t cc = column(x0 - NS + i);
+            const double *slot = st.column(i);
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[q] = s[cc + q * L + y];
+            for (int q = 0; q < 9; ++q) f[q] = slot[q * 64 + lane];
         }
+        if (i + kStageAhead < iters)
+            st.fetch(s, column(x0 - NS + i + kStageAhead), L, (i + kStageAhead) % kStageBufs);
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             if (i < 2 * k) break;  // level k has no column yet
@@ -334,17 +345,17 @@ extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst
 }
 
 // Steps per pass of plbm_run_op when the caller has not chosen, where the grid gives at
-// least 2048 wavefronts (FP64 storage; else single steps): 3, except 2 for the 9-point
-// operators with the TRT / RR collisions, whose 3-step pass no longer fits two
-// wavefronts per SIMD (measured at 8192^2: FVM + TRT 1.49 ms per step in passes of 2,
-// 1.83 in passes of 3, 2.09 in single steps).
+// least 2048 wavefronts (FP64 storage; else single steps): 3 for the Sofonea operator, 2
+// for the 9-point Bardow operators, whose 3-step pass no longer fits two wavefronts per
+// SIMD (measured at 8192^2, ms per step in passes of 1 / 2 / 3: FVM + BGK 2.06 / 1.71 /
+// 1.89, FDM + TRT 2.02 / 1.05 / 1.30, Sofonea + BGK 1.94 / 1.00 / 0.69).
 static int fd_steps_per_pass(const plbm_lattice *h, int op, int model) {
+    (void)model;
     if (h->fd_ns > 0) return h->fd_ns;
     if (h->storage != PLBM_F64) return 1;
     const long waves = (long)((h->ny + 55) / 56) * ((h->nx + kMultiXS - 1) / kMultiXS);
     if (waves < 2048) return 1;
-    if (op == PLBM_STREAM_FDM_SOFONEA || model == PLBM_BGK || model == PLBM_BGK_SPLIT) return 3;
-    return 2;
+    return op == PLBM_STREAM_FDM_SOFONEA ? 3 : 2;
 }
 
 extern "C" int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps) {
