@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000,
                     help="steps of the batched uneven-size sweep (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--no-rr", action="store_true", help="skip the RR collision leg")
+    ap.add_argument("--no-rr", action="store_true",
+                    help="skip the leg of the other collisions (BGK, RR)")
     ap.add_argument("--rr-acc-steps", type=int, default=1000,
-                    help="steps of the RR FP32-vs-FP64 error check (0 to skip)")
+                    help="steps of that leg's FP32-vs-FP64 error check (0 to skip)")
     ap.add_argument("--next-steps", type=int, default=50,
                     help="timed steps of the DUGKS and FV/FD streaming legs (0 skips them)")
     return ap.parse_args()
@@ -162,8 +163,8 @@ def gpu_leg(args):
 
     copy_gbs = P.copy_bandwidth(4 << 30, 20)
     extras = {
-        "rr_forms": None if args.no_rr else rr_leg(P, torch, args, stream, omega,
-                                                   (d_rho, d_ux, d_uy)),
+        "collisions": None if args.no_rr else collisions_leg(P, torch, args, stream, omega,
+                                                             (d_rho, d_ux, d_uy)),
         "schemes": schemes_leg(P, torch, args, stream, (d_rho, d_ux, d_uy))
         if args.next_steps > 0 else None,
         "batched_sweep": batch_leg(P, torch, args) if args.batch_steps > 0 else None,
@@ -172,40 +173,45 @@ def gpu_leg(args):
     return results, copy_gbs, dev_name, extras
 
 
-def rr_leg(P, torch, args, stream, omega, fields):
-    """The recursive-regularized collision (src/collision_regularized.F90) on the same
-    8192^2 start: FP64 and FP32-shifted MLUPS through plbm_run (autotuned), and the
-    FP32-shifted velocity error against the FP64 run after rr_acc_steps steps."""
+def collisions_leg(P, torch, args, stream, omega, fields):
+    """The other collision models on the same 8192^2 start: BGK (src/collision_bgk.F90)
+    and recursive-regularized (src/collision_regularized.F90), FP64 and FP32-shifted MLUPS
+    through plbm_run (autotuned), and the FP32-shifted velocity error against the FP64 run
+    after rr_acc_steps steps."""
     n = args.n
     cells = n * n
-    out = {"grid": [n, n], "steps": args.steps}
-    macros = {}
-    for storage in ("f64", "f32s"):
-        lat = P.Lattice(n, n, 2, storage)
-        lat.set_stream(stream)
-        lat.set_params(omega, MAGIC)
-        lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
-        torch.cuda.synchronize()
-        lat.autotune("rr", 0, reps=5)
-        ms, _ = timed_run(P, torch, lat, "rr", args.steps, args.warmup)
-        out[storage] = {"ms_per_step": round(ms, 5), "mlups": round(cells / ms / 1e3, 1),
-                        "steps_per_pass": lat.get_steps_per_launch("rr")}
-        if args.rr_acc_steps > 0:
+    out = {}
+    for model in ("bgk", "rr"):
+        res = {"grid": [n, n], "steps": args.steps}
+        macros = {}
+        for storage in ("f64", "f32s"):
+            lat = P.Lattice(n, n, 2, storage)
+            lat.set_stream(stream)
+            lat.set_params(omega, MAGIC)
             lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
-            iold, _ = lat.run("rr", args.rr_acc_steps, 0, 1)
-            m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
-            lat.macros_dev(iold, *(t.data_ptr() for t in m))
             torch.cuda.synchronize()
-            macros[storage] = m
-        lat.close()
-        torch.cuda.empty_cache()
-    out["f32s_speedup_vs_f64"] = round(out["f32s"]["mlups"] / out["f64"]["mlups"], 3)
-    if macros:
-        (r64, u64, v64), (r, u, v) = macros["f64"], macros["f32s"]
-        den = torch.sum(u64 * u64 + v64 * v64)
-        out["f32s"]["vel_rel_l2_vs_f64"] = torch.sqrt(
-            torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
-        out["f32s"]["acc_steps"] = args.rr_acc_steps
+            lat.autotune(model, 0, reps=5)
+            ms, _ = timed_run(P, torch, lat, model, args.steps, args.warmup)
+            res[storage] = {"ms_per_step": round(ms, 5), "mlups": round(cells / ms / 1e3, 1),
+                            "steps_per_pass": lat.get_steps_per_launch(model)}
+            if args.rr_acc_steps > 0:
+                lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
+                iold, _ = lat.run(model, args.rr_acc_steps, 0, 1)
+                m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
+                lat.macros_dev(iold, *(t.data_ptr() for t in m))
+                torch.cuda.synchronize()
+                macros[storage] = m
+            lat.close()
+            torch.cuda.empty_cache()
+        res["f32s_speedup_vs_f64"] = round(res["f32s"]["mlups"] / res["f64"]["mlups"], 3)
+        if macros:
+            (r64, u64, v64), (r, u, v) = macros["f64"], macros["f32s"]
+            den = torch.sum(u64 * u64 + v64 * v64)
+            res["f32s"]["vel_rel_l2_vs_f64"] = torch.sqrt(
+                torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
+            res["f32s"]["acc_steps"] = args.rr_acc_steps
+        del macros
+        out[model] = res
     return out
 
 
@@ -494,7 +500,7 @@ def main():
             for k, v in results.items()},
         "f32s_speedup_vs_f64": round(results["f32s"]["mlups"] / f64["mlups"], 3),
         "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
-        "rr_collision": extras["rr_forms"],
+        "other_collisions": extras["collisions"],
         "batched_sweep": extras["batched_sweep"],
         "adjoint": extras["adjoint"],
         "schemes": extras["schemes"],
