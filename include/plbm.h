@@ -239,7 +239,8 @@ int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap
 /* Temporal blocking of plbm_run: steps advanced per pass over HBM (1 = the single-step
  * kernel; 2..8 = that many fused steps per launch, each wavefront walking a segment of
  * columns with the intermediate states in registers; 0 = chosen from the grid size: 8
- * when the grid gives at least 2048 wavefronts of 48 x 64 cells, else 1).  In F64 storage the results are bit-identical for every setting;
+ * when the default work split gives at least 512 wavefronts, about 512^2 and larger,
+ * else 1).  In F64 storage the results are bit-identical for every setting;
  * the shifted storages agree to roundoff (their collisions allow FMA contraction, which
  * the compiler places per kernel).  plbm_run keeps the reference's slot contents (latest
  * in *iold, the state one step earlier in *inew); with passes of more than one step it
@@ -253,8 +254,8 @@ int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst);
 int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *steps);
 /* Work split of a pass (engine extension, no counterpart in the reference): each band of
  * 56 (F64 / F64S) or 48 (F32S / F16S) rows is cut into `segments` column ranges of nearly
- * equal length, one wavefront each; 0 = chosen from the grid (64-column segments, longer
- * on large grids).  Results do not depend on it.  The get variant reports the setting. */
+ * equal length, one wavefront each; 0 = chosen from the grid (64 segments, each of at
+ * least 8 columns).  Results do not depend on it.  The get variant reports the setting. */
 int plbm_set_pass_segments(plbm_lattice *h, int segments);
 int plbm_get_pass_segments(const plbm_lattice *h, int *segments);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from

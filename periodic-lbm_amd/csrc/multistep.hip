@@ -31,8 +31,6 @@
 // population) for NS lattice updates; the halo re-reads (64/VR in y, (XS+2 NS)/XS in x)
 // are the overhead.  Work per output cell and pass: NS collisions plus the halo ones.
 #include <algorithm>
-#include <map>
-#include <mutex>
 #include <string>
 #include <utility>
 
@@ -56,21 +54,16 @@ __global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
     if (x1 > x0) multi_wave<MODEL, ST, NS>(a, band, x0, x1 - x0);
 }
 
-// Wavefronts the GPU holds at once for a kernel of 256-thread workgroups.
-static int resident_waves(const void *kernel, int device) {
-    static std::mutex mu;
-    static std::map<std::pair<const void *, int>, int> cache;
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find({kernel, device});
-    if (it != cache.end()) return it->second;
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-        return 0;
-    return cache[{kernel, device}] = per_cu * cus * 4;
-}
-
 int max_steps_per_pass(int) { return kMaxPassSteps; }
+
+// Column segments per row band of a pass when the caller has not chosen: 64, each of at
+// least 8 columns.  Measured with 8 steps per pass (TRT, GLUPS for 48 / 64 / 96 segments
+// per band; FP64, FP32, FP16 storage): 8192^2 181 / 191 / 184, 351 / 366 / 358, 360 / 378
+// / 365; 4096^2 168 / 174 / 168, 305 / 326 / 316, 273 / 322 / 333; FP64 2048^2 (32 / 64
+// / 128) 145 / 154 / 136, 1024^2 84 / 102 / 94, 512^2 (16 / 32 / 64) 24 / 38 / 53.  With
+// fewer wavefronts than resident slots the short segments still pay: the pass is bound by
+// each wavefront's sequential walk.
+int default_pass_segments(int nx) { return std::max(1, std::min(64, nx / 8)); }
 
 int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst) {
     StepArgs a{};
@@ -94,15 +87,8 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst)
                 if constexpr (NS <= max_steps_per_pass<SS>()) {
                     constexpr int vr = multi_vr<SS, NS>();
                     const int waves_y = (h->ny + vr - 1) / vr;
-                    const void *kern = reinterpret_cast<const void *>(&k_multi<MM, SS, NS>);
-                    // Segments of kMultiXS columns, or fewer and longer ones where that still
-                    // gives 4 wavefronts per resident slot (8192^2 FP64 TRT, 4 steps: 64-column
-                    // strips 1.96 ms per pass, 149-column segments 1.92 ms; one segment per
-                    // resident slot, 630 columns, 1.96 ms).
-                    int m = (h->nx + kMultiXS - 1) / kMultiXS;
-                    const long long res = 4LL * resident_waves(kern, h->device);
-                    if (res > 0) m = (int)std::min<long long>(m, std::max(1LL, res / waves_y));
-                    if (h->pass_segs > 0) m = std::min(h->pass_segs, h->nx);
+                    const int m = h->pass_segs > 0 ? std::min(h->pass_segs, h->nx)
+                                                   : default_pass_segments(h->nx);
                     const long long nw = (long long)waves_y * m;
                     const dim3 grid((unsigned)((nw + 3) / 4));
                     hipLaunchKernelGGL((k_multi<MM, SS, NS>), grid, dim3(256), 0, h->stream, a,
@@ -170,14 +156,15 @@ int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *in
 
 namespace plbm {
 
-// Steps per pass when the caller has not chosen: 8 where the grid gives at least 2048
-// wavefronts of 48 rows x 64 columns (8 per CU), else single steps (a small grid cannot
-// fill the chip with strips).  At 8192^2 TRT 8 is the fastest depth for every storage
-// (FP64: 0.36 ms per step against 0.48 at 4).
+// Steps per pass when the caller has not chosen: 8 where the default work split gives at
+// least 512 wavefronts, else single steps.  TRT, 8 steps per pass against single steps,
+// GLUPS (FP64): 512^2 52.6 / 31.9, 1024^2 101.5 / 35.4, 2048^2 153.6 / 40.8, 4096^2 187
+// / 42.5; 320^2 (35 wavefronts of 64 columns) 36 against 5 us per step.  At 8192^2 8 is
+// the fastest depth for every storage.
 int default_steps_per_launch(const plbm_lattice *h) {
     const int vr = multi_rows(storage_bytes(h->storage), kMaxPassSteps);
-    const long waves = (long)((h->ny + vr - 1) / vr) * ((h->nx + kMultiXS - 1) / kMultiXS);
-    return waves >= 2048 ? kMaxPassSteps : 1;
+    const long waves = (long)((h->ny + vr - 1) / vr) * default_pass_segments(h->nx);
+    return waves >= 512 ? kMaxPassSteps : 1;
 }
 
 }  // namespace plbm

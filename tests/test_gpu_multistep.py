@@ -139,17 +139,19 @@ def test_pass_segments_do_not_change_results(P, gpu, nx, ny, segments):
         lat.set_pass_segments(-1)
 
 
-def test_automatic_segments_large_grid(P, gpu):
-    """8192 x 4096 FP64 TRT, 5 steps: the automatic work split (about 110 segments of 74-75
-    columns per band, more than one strip of 64) against single steps."""
-    nx, ny, nsteps = 8192, 4096, 5
+@pytest.mark.parametrize("nx,ny", [(8192, 4096), (1024, 1024), (1000, 530)])
+def test_automatic_work_split(P, gpu, nx, ny):
+    """FP64 TRT, 5 steps, the automatic depth (passes of 8) and work split (64 segments per
+    band: 128 columns at 8192, 16 at 1024, 15-16 at 1000) against single steps."""
+    nsteps = 5
     print(f"case segments auto {nx}x{ny}", flush=True)
     rho, ux, uy = fields(nx, ny, 9)
     out = []
-    for ns in (1, 4):
+    for ns in (1, 0):
         lat = P.Lattice(nx, ny, 2, "f64")
         lat.set_params(1.0 / (3 * 0.01 + 0.5), 0.25)
         lat.set_steps_per_launch("trt", ns)
+        assert lat.get_steps_per_launch("trt") == (ns or 8)
         lat.equilibrium_init(0, rho, ux, uy)
         i, n = lat.run("trt", nsteps, 0, 1)
         out.append((i, lat.download_f(i)))
