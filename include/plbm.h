@@ -306,7 +306,7 @@ int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps);
 int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps);
 /* Columns each wavefront of a pass walks (engine extension, no counterpart in the
  * reference): a case is cut into segments of this many columns.  Default 64; the
- * autotuner tries 64, 128 and 256.  Results do not depend on it. */
+ * autotuner tries 16 to 256.  Results do not depend on it. */
 int plbm_batch_set_pass_columns(plbm_batch *b, int columns);
 int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns);
 /* Autotuner: time `steps` (even) steps of every plan candidate (tile shapes, steps per

@@ -824,7 +824,7 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
             }
     const int max_ns = max_steps_per_pass(b->storage);
     for (int ns = 2; ns <= max_ns; ns += (ns < 4 ? 1 : 2))
-        for (int xs : {kMultiXS, 2 * kMultiXS, 4 * kMultiXS})
+        for (int xs : {16, 32, 64, 128, 256})
             for (int res : {0, 4096, 16384, 36864})
                 for (int rb : {256, 1024}) {
                     if (res == 0 && rb != 256) continue;
