@@ -4,10 +4,10 @@
 // lattice update), so it is bound by HBM at one step per pass.  Here one pass advances the
 // lattice NS steps (NS = 2..8) and reads / writes every population once per pass:
 //
-//  * A wavefront owns 64 consecutive rows y and walks a segment of XS columns along x
-//    (64 columns, or longer on large grids).  Of its rows, VR produce output; NS rows on
-//    each side are halo (VR + 2 NS <= 64).  VR is a multiple of 8 so that each wave's
-//    stores start and end on 64-byte boundaries.
+//  * A wavefront owns 64 consecutive rows y and walks a segment of columns along x (by
+//    default 1/64 of the width, at least 8 columns).  Of its rows, VR produce output; NS
+//    rows on each side are halo (VR + 2 NS <= 64).  VR is a multiple of 8 so that each
+//    wave's stores start and end on 64-byte boundaries.
 //  * Level k (k = 0 .. NS-1) keeps the pull-ready values of f^{n+k} for the last three
 //    columns in registers.  Loading column c gives level 0 at c; level k+1 at column
 //    c - k - 1 follows from level k at columns c - k, c - k - 1, c - k - 2 (the pull
@@ -28,8 +28,8 @@
 //    8-step pass at 8192^2).
 //
 // Algorithmic traffic per pass: 18 s bytes per cell (one read, one write of each
-// population) for NS lattice updates; the halo re-reads (64/VR in y, (XS+2 NS)/XS in x)
-// are the overhead.  Work per output cell and pass: NS collisions plus the halo ones.
+// population) for NS lattice updates; the halo re-reads (64/VR in y, 2 NS columns per
+// segment in x) are the overhead.  Work per output cell and pass: NS collisions plus the halo ones.
 #include <algorithm>
 #include <string>
 #include <utility>
