@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256)
     k_batch_multi(const BatchCase *__restrict__ cases, const BatchTile *__restrict__ waves,
                   int nwaves, void *arena, int parity, float scale, float inv_scale) {
     using T = typename Traits<ST>::T;
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (gw >= nwaves) return;  // the whole wavefront
     const BatchTile w = waves[gw];
     const BatchCase c = cases[w.c];

@@ -47,7 +47,8 @@ int launch_step(plbm_lattice *h, int model, const void *src, void *dst);
 // are in L2).
 template <int MODEL, int ST, int NS>
 __global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // wave-uniform: the column walk's indices and addresses are then scalar
+    const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (gw >= waves_y * m) return;  // the whole wavefront
     const int band = gw % waves_y, j = gw / waves_y;
     const int x0 = (int)((long long)a.nx * j / m), x1 = (int)((long long)a.nx * (j + 1) / m);

@@ -167,6 +167,12 @@ struct ColumnStage {
     }
 };
 
+// Lane of a staged slot that lane `lane` reads for the pull in y of a population with
+// velocity cy_: the neighbouring row, clamped at the wavefront's edges (halo lanes).
+__device__ __forceinline__ int stage_pull_lane(int cy_, int lane) {
+    return cy_ > 0 ? max(lane - 1, 0) : (cy_ < 0 ? min(lane + 1, 63) : lane);
+}
+
 // This wavefront's LDS staging buffers (four wavefronts per workgroup).
 __device__ __forceinline__ double *stage_buffers() {
     __shared__ double stage_mem[kStageWaves][kStageBufs][kStageSlots][64];
@@ -229,8 +235,12 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     constexpr bool kStage = sizeof(T) == 8;
     T nxt[9];
     ColumnStage st{};
-    if constexpr (kStage)
+    int pull_lane[3];  // staged slot lanes of the level-0 pull for cy = -1, 0, +1
+    if constexpr (kStage) {
         st.init(stage_buffers(), reinterpret_cast<const double *>(s), L, ny, VR * wy - NS, lane);
+#pragma unroll
+        for (int c = -1; c <= 1; ++c) pull_lane[c + 1] = stage_pull_lane(c, lane);
+    }
     auto fetch = [&](int c, int buf) {
         const size_t cc = column(c);
         if constexpr (kStage) {
@@ -254,10 +264,8 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
                 W[0][2][q] = W[0][1][q];
                 W[0][1][q] = W[0][0][q];
                 // the pull in y reads the neighbouring row's slot (edge lanes are halo)
-                const int src_lane = cy(q) > 0 ? max(lane - 1, 0)
-                                               : (cy(q) < 0 ? min(lane + 1, 63) : lane);
-                W[0][0][q] = load_pop<ST>(reinterpret_cast<const T *>(slot + q * 64 + src_lane),
-                                          a.inv_scale);
+                W[0][0][q] = load_pop<ST>(
+                    reinterpret_cast<const T *>(slot + q * 64 + pull_lane[cy(q) + 1]), a.inv_scale);
             }
         } else {
 #pragma unroll

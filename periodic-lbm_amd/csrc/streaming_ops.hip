@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256)
     constexpr int VR = 56;
     static_assert(VR + 2 * NS <= 64, "halo rows do not fit in a wavefront");
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (gw >= nwaves) return;  // the whole wavefront
     const int nx = a.nx, ny = a.ny;
     const int wy = gw % waves_y, xsi = gw / waves_y;
