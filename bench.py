@@ -483,8 +483,8 @@ def main():
         "measured_copy_gbs": round(copy_gbs, 1),
         "frac_of_measured_copy": round(achieved / copy_gbs, 4),
         "single_step_frac_of_measured_copy": round(f64["single_step_gbs"] / copy_gbs, 4),
-        # the baseline's own "% roofline": MLUPS x one step's 144 B per lattice update over
-        # the measured copy bandwidth (above 1 once several steps share one sweep)
+        # "% roofline" in the lattice-update sense: MLUPS x one step's 144 B per lattice
+        # update over the measured copy bandwidth (above 1 once several steps share a sweep)
         "lu_bytes_rate_over_measured_copy": round(
             f64["mlups"] * 1e6 * BYTES_PER_LU["f64"] / 1e9 / copy_gbs, 3),
         "storage_forms": {
