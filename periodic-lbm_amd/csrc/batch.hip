@@ -360,18 +360,17 @@ static int set_cfg(plbm_batch *b, const BatchCfg &cfg) {
     return PLBM_OK;
 }
 
-// Pass depths (each <= ns) summing to nsteps whose count has the parity of nsteps.
+// Pass depths (each <= ns) summing to nsteps whose count has the parity of nsteps: the
+// fewest passes of that parity, the steps spread evenly over them (58 steps at ns = 6:
+// 6 passes of 6 and 4 of 5, rather than 9 of 6, one of 3 and one of 1; a short pass costs
+// nearly a full sweep).
 static std::vector<int> pass_depths(int ns, int nsteps) {
     std::vector<int> d;
-    for (int left = nsteps; left > 0; left -= d.back()) d.push_back(std::min(ns, left));
-    if ((int)(d.size() % 2) != nsteps % 2) {
-        for (size_t i = 0; i < d.size(); ++i)
-            if (d[i] >= 2) {
-                d[i] -= 1;
-                d.insert(d.begin() + (long)i + 1, 1);
-                break;
-            }
-    }
+    if (nsteps <= 0) return d;
+    int p = (nsteps + ns - 1) / ns;
+    if (p % 2 != nsteps % 2) ++p;  // still <= nsteps: p = nsteps has the parity of nsteps
+    const int q = nsteps / p, r = nsteps % p;
+    for (int i = 0; i < p; ++i) d.push_back(q + (i < r ? 1 : 0));
     return d;
 }
 
@@ -758,12 +757,14 @@ int plbm_batch_run(plbm_batch *b, int nsteps) {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
     PLBM_REQUIRE(b->params_set, PLBM_ERR_STATE, "call plbm_batch_set_params first");
-    // Replay a graph of up to kChunk steps; the chunk is even, so parity is preserved
-    // and one graph serves every full chunk.
-    constexpr int kChunk = 100;
+    // Replay graphs of up to about 100 steps.  The chunk is even, so parity is preserved
+    // and one graph serves every full chunk; with passes of ns steps it is a multiple of
+    // 2 ns (96 at ns = 6), so that a full chunk is whole passes of ns steps.
+    const int unit = b->cfg.ns > 1 ? 2 * b->cfg.ns : 2;
+    const int chunk = std::max(unit, 100 / unit * unit);
     int done = 0;
     while (done < nsteps) {
-        const int n = std::min(kChunk, nsteps - done);
+        const int n = std::min(chunk, nsteps - done);
         hipGraphExec_t ex;
         PLBM_TRY(graph_for(b, n, b->parity, &ex));
         PLBM_HIP(hipGraphLaunch(ex, b->stream));

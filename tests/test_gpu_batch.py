@@ -85,6 +85,17 @@ def test_batch_autotune_and_reductions(P, gpu):
                                                  ("f16s", "trt", 6, 64)])
 @pytest.mark.parametrize("nsteps", [7, 30])
 def test_batch_passes(P, gpu, storage, model, ns, xs, nsteps):
+    check_batch_passes(P, storage, model, ns, xs, nsteps)
+
+
+@pytest.mark.parametrize("ns", [5, 6, 7])
+def test_batch_passes_over_several_chunks(P, gpu, ns):
+    """250 steps: graph chunks of 2 ns x k steps (100, 96, 98) and a shorter last chunk,
+    each split into passes of nearly equal depth; FP64 TRT equals the single lattice."""
+    check_batch_passes(P, "f64", "trt", ns, 64, 250)
+
+
+def check_batch_passes(P, storage, model, ns, xs, nsteps):
     """Temporally blocked passes for the tiled cases: FP64 cases equal the single lattice
     bit for bit, shifted storages to roundoff; the resident cases keep stepping one step
     at a time in parallel, and the two agree on the buffer holding the latest state."""
