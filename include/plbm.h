@@ -310,8 +310,9 @@ int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps);
 int plbm_batch_set_pass_columns(plbm_batch *b, int columns);
 int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns);
 /* Autotuner: time `steps` (even) steps of every plan candidate (tile shapes, steps per
- * pass, resident threshold) on a scratch copy of the arena and keep the fastest;
- * *best_ms (may be NULL) = its time per step. */
+ * pass, resident threshold; a plan with passes of ns steps over `steps` rounded up to a
+ * multiple of 2 ns, so that it runs whole passes) on a scratch copy of the arena twice
+ * and keep the fastest; *best_ms (may be NULL) = its time per step. */
 int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms);
 
 /* ---------------------------------------------------------------- adjoint */

@@ -845,10 +845,14 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
         int nt = 0, nr = 0;
         status = build_plan(b, c, &t, &nt, &r, &nr);
         if (status != PLBM_OK) break;
+        // a pass plan is timed on whole passes of its depth, as plbm_batch_run's chunks
+        // run it (at least `steps`, rounded up to a multiple of 2 ns; the even split of a
+        // shorter count would time shallower passes, e.g. 10 steps at ns = 8 as two of 5)
+        const int n = c.ns > 1 ? 2 * c.ns * ((steps + 2 * c.ns - 1) / (2 * c.ns)) : steps;
         double tmin = 1e300;
         for (int trial = 0; trial < 2 && status == PLBM_OK; ++trial) {
             (void)hipEventRecord(e0, b->stream);
-            status = enqueue_steps(b, scratch, t, nt, r, nr, c, 0, steps);
+            status = enqueue_steps(b, scratch, t, nt, r, nr, c, 0, n);
             (void)hipEventRecord(e1, b->stream);
             if (status == PLBM_OK && hipEventSynchronize(e1) != hipSuccess) {
                 set_error("batch autotune: event synchronisation failed");
@@ -856,7 +860,7 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
             }
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            tmin = std::min(tmin, double(ms) / steps);
+            tmin = std::min(tmin, double(ms) / n);
         }
         if (t) (void)hipFree(t);
         if (r) (void)hipFree(r);
