@@ -124,8 +124,9 @@ int plbm_step(plbm_lattice *h, int model, int src, int dst);
 /* perform_lbm_step (src/periodic_lbm.f90:15-29) nsteps times with the reference's
  * buffer bookkeeping: on entry *iold / *inew are the current ids; each step reads *iold,
  * writes *inew, then swaps them.  Runs in passes of several steps where that is faster
- * (plbm_set_steps_per_launch); the slot contents afterwards are the reference's.
- * Asynchronous. */
+ * (plbm_set_steps_per_launch, plbm_set_pass_lag_buffer); the slot contents afterwards are
+ * the reference's, but the slots' device storage may have moved (fetch plbm_buffer_ptr
+ * again).  Asynchronous. */
 int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew);
 /* Unfused operators for the procedure-pointer slots grid%streaming / grid%collision:
  * lbm_stream (src/periodic_lbm.f90:32-129) src -> dst; collide_* in place on buf. */
@@ -258,6 +259,14 @@ int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *steps);
  * least 8 columns).  Results do not depend on it.  The get variant reports the setting. */
 int plbm_set_pass_segments(plbm_lattice *h, int segments);
 int plbm_get_pass_segments(const plbm_lattice *h, int *segments);
+/* Third population buffer of plbm_run / plbm_run_op in passes (engine extension, no
+ * counterpart in the reference).  With it (enable = 1, the default) the last pass of a run
+ * also writes the state one step before the latest, which the other slot must hold
+ * (update_macros reads it), into a third buffer allocated on the first such run; the
+ * three buffers' storage rotates among the two slots and the spare.  With enable = 0 the
+ * buffer is freed and runs end on a single-step launch instead (one more sweep over
+ * HBM per run, two buffers of memory).  Results are identical either way. */
+int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
  * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
  * of 2..8 steps against it per lattice update; keep the fastest for this lattice and
@@ -321,8 +330,8 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms);
  * f^0 = feq(rho0, u0), f^{n+1} = collide_bgk(lbm_stream(f^n)) with the engine's fused
  * kernel; loss L = 1/2 sum_cells |u(f^N) - u*|^2, u = j/rho.  States are checkpointed
  * every checkpoint_every steps; the reverse sweep recomputes each segment and runs the
- * fused reverse kernel.  storage: PLBM_F64, PLBM_F64S (FP64 adjoint) or PLBM_F32S (FP32
- * forward and adjoint).  All field pointers are device pointers to (ny, nx) Fortran-order
+ * fused reverse kernel.  storage: PLBM_F64, PLBM_F64S or PLBM_F32S (FP32
+ * forward steps and checkpoints); the adjoint state is FP64 for every storage.  All field pointers are device pointers to (ny, nx) Fortran-order
  * FP64 arrays; the initial fields must stay valid until the backward call. */
 typedef struct plbm_adjoint plbm_adjoint;
 int plbm_adjoint_create(int nx, int ny, int storage, int nsteps, int checkpoint_every,

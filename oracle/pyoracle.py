@@ -223,6 +223,7 @@ class RefLib:
         L.ref_set_streaming.argtypes = [ctypes.c_void_p, _i]
         L.ref_streaming.argtypes = [ctypes.c_void_p]
         L.ref_perform_step.argtypes = [ctypes.c_void_p, _i]
+        L.ref_perform_triple_step.argtypes = [ctypes.c_void_p, _i]
         L.ref_set_dugks.argtypes = [ctypes.c_void_p]
         L.ref_perform_dugks_step.argtypes = [ctypes.c_void_p, _i]
 
@@ -315,6 +316,10 @@ class RefGrid:
 
     def perform_step(self, n=1):
         self.ref.lib.ref_perform_step(self.h, n)
+
+    def triple_step(self, n=1):
+        """perform_triple_step (src/fvm_bardow.F90:322-340) n times (nf = 3 grids)."""
+        self.ref.lib.ref_perform_triple_step(self.h, n)
 
     def collide(self):
         self.ref.lib.ref_collide(self.h)

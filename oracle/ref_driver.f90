@@ -12,7 +12,8 @@
 !   collide_trt, magic_number, lambda_d  src/collision_trt.F90:24-160
 !   collide_rr                           src/collision_regularized.F90:21-202
 !   taylor_green_t%eval                  src/benchmarks/taylor_green.f90:31-84
-!   shear_layer_t%eval                   src/benchmarks/shear_layer.f90:123-150
+!   shear_layer_t eval_shear_layer_grid  src/benchmarks/shear_layer.f90:25-52
+!   perform_triple_step                  src/fvm_bardow.F90:322-340
 !   perform_dugks_step, dugks_collide,
 !   dugks_stream                         src/periodic_dugks.F90:25-299
 module ref_driver
@@ -209,6 +210,17 @@ contains
       end do
    end subroutine
 
+   subroutine ref_perform_triple_step(h, nsteps) bind(c, name="ref_perform_triple_step")
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+      type(grid_box), pointer :: p
+      integer :: s
+      p => unbox(h)
+      do s = 1, nsteps
+         call perform_triple_step(p%grid)
+      end do
+   end subroutine
+
    subroutine ref_set_dugks(h) bind(c, name="ref_set_dugks")
       type(c_ptr), value :: h
       type(grid_box), pointer :: p
@@ -320,7 +332,7 @@ contains
       sl%k = k
       sl%delta = delta
       ! `eval` binds eval_shear_layer_unstructured; the grid form is its own binding
-      ! (src/benchmarks/shear_layer.f90:115-116)
+      ! (src/benchmarks/shear_layer.f90:17-18)
       call sl%eval_shear_layer_grid(nx, ny, rho, ux, uy)
    end subroutine
 

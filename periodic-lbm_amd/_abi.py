@@ -117,6 +117,7 @@ SIGNATURES = {
     "plbm_get_steps_per_launch": (_i, [_vp, _i, _ip]),
     "plbm_set_pass_segments": (_i, [_vp, _i]),
     "plbm_get_pass_segments": (_i, [_vp, _ip]),
+    "plbm_set_pass_lag_buffer": (_i, [_vp, _i]),
     "plbm_dugks_collide": (_i, [_vp, _i, _i, _d, _d]),
     "plbm_dugks_stream": (_i, [_vp, _i, _i, _d]),
     "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),

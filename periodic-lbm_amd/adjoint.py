@@ -36,6 +36,7 @@ class Adjoint:
                                         checkpoint_every, ctypes.byref(h)))
         self.h = h
         self.nx, self.ny, self.nsteps = nx, ny, nsteps
+        self.generation = 0  # forward() calls so far; backward() differentiates the last
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -64,6 +65,7 @@ class Adjoint:
         check(lib().plbm_adjoint_forward_dev(self.h, rho0.data_ptr(), ux0.data_ptr(),
                                              uy0.data_ptr(), float(omega), uxt.data_ptr(),
                                              uyt.data_ptr(), ctypes.byref(loss)))
+        self.generation += 1
         return loss.value
 
     def backward(self):
@@ -71,6 +73,9 @@ class Adjoint:
         import torch
         dev = torch.device("cuda")
         g = [torch.empty((self.nx, self.ny), dtype=torch.float64, device=dev) for _ in range(3)]
+        # the engine writes the gradients on its own stream: blocks that the caching
+        # allocator just handed out may still be read by queued work on torch's stream
+        torch.cuda.current_stream().synchronize()
         gom = ctypes.c_double()
         check(lib().plbm_adjoint_backward_dev(self.h, g[0].data_ptr(), g[1].data_ptr(),
                                               g[2].data_ptr(), ctypes.byref(gom)))
@@ -90,12 +95,18 @@ def _make_function():
             ux0c, uy0c = ux0.detach().contiguous(), uy0.detach().contiguous()
             loss = engine.forward(rho0, ux0c, uy0c, 1.0 / t, uxt.contiguous(), uyt.contiguous())
             ctx.engine = engine
+            ctx.generation = engine.generation
             ctx.tau = t
             ctx.keep = (rho0, ux0c, uy0c)  # the engine reads them again in backward
             return torch.tensor(loss, dtype=torch.float64, device=ux0.device)
 
         @staticmethod
         def backward(ctx, gl):
+            if ctx.engine.generation != ctx.generation:
+                raise RuntimeError(
+                    "velocity_misfit: the Adjoint engine ran another forward since this loss "
+                    "was computed; its checkpoints belong to the newer call (use one engine "
+                    "per loss that is still to be differentiated)")
             _, gux, guy, gomega = ctx.engine.backward()
             gtau = -gomega / (ctx.tau * ctx.tau)
             g = gl.to(torch.float64)

@@ -61,34 +61,36 @@ __device__ __forceinline__ void bgk_jacobian_t(const R g_unshifted_rho, const R 
                 omega * (A + (Bx * (R(cx(j)) - ux) + By * (R(cy(j)) - uy)) * inv);
 }
 
-// Unshifted populations and macros of one stored cell (pulled or not), arithmetic in R.
-template <int ST>
-__device__ __forceinline__ void cell_state(const typename Traits<ST>::R gs[9],
-                                           typename Traits<ST>::R &rho,
-                                           typename Traits<ST>::R &ux,
-                                           typename Traits<ST>::R &uy,
-                                           typename Traits<ST>::R &drho) {
-    using R = typename Traits<ST>::R;
-    R d, jx, jy;
-    moments_shifted<R>(gs, d, jx, jy);  // plain sums: valid for shifted and unshifted
-    if constexpr (Traits<ST>::shifted) {
+// Macros of one stored cell (pulled or not) from its populations g (deviations
+// g_q = f_q - w_q for the shifted storages), in the adjoint's FP64 arithmetic.
+template <bool SHIFTED>
+__device__ __forceinline__ void cell_state(const double gs[9], double &rho, double &ux,
+                                           double &uy, double &drho) {
+    double d, jx, jy;
+    moments_shifted<double>(gs, d, jx, jy);  // plain sums: valid for shifted and unshifted
+    if constexpr (SHIFTED) {
         drho = d;
-        rho = R(1) + d;
+        rho = 1.0 + d;
     } else {
         rho = d;
-        drho = d - R(1);
+        drho = d - 1.0;
     }
     ux = jx / rho;
     uy = jy / rho;
 }
 
+// The adjoint state lambda, mu and the omega sensitivity are FP64 for every storage: the
+// forward states of FP32 storage convert to FP64 exactly, so the reverse sweep is the
+// exact adjoint of the FP64 step evaluated along the FP32 trajectory, and its own
+// rounding (500 reverse steps of FP32 would add about 1e-5 relative to the gradient)
+// drops out.
+
 // lambda^N and the loss of the final state.
 template <int ST>
 __global__ void __launch_bounds__(256)
     k_adj_final(const void *fN, int nx, int ny, int ldd, float inv_scale, const double *uxt,
-                const double *uyt, typename Traits<ST>::R *lam, double *partials) {
+                const double *uyt, double *lam, double *partials) {
     using T = typename Traits<ST>::T;
-    using R = typename Traits<ST>::R;
     __shared__ double sh[256];
     double acc = 0.0;
     const size_t n = (size_t)nx * ny;
@@ -97,17 +99,18 @@ __global__ void __launch_bounds__(256)
         const int x = (int)(i / ny), y = (int)(i % ny);
         const T *s = static_cast<const T *>(fN);
         const size_t c = pop_index(0, x, y, ldd);
-        R g[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = load_pop<ST>(s + c + (size_t)q * ldd, inv_scale);
-        R rho, ux, uy, drho;
-        cell_state<ST>(g, rho, ux, uy, drho);
-        const R ex = ux - R(uxt[i]), ey = uy - R(uyt[i]);
-        acc += 0.5 * (double(ex) * double(ex) + double(ey) * double(ey));
-        const R inv = R(1) / rho;
+        double g[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q)
-            lam[c + (size_t)q * ldd] = (ex * (R(cx(q)) - ux) + ey * (R(cy(q)) - uy)) * inv;
+            g[q] = double(load_pop<ST>(s + c + (size_t)q * ldd, inv_scale));
+        double rho, ux, uy, drho;
+        cell_state<Traits<ST>::shifted>(g, rho, ux, uy, drho);
+        const double ex = ux - uxt[i], ey = uy - uyt[i];
+        acc += 0.5 * (ex * ex + ey * ey);
+        const double inv = 1.0 / rho;
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            lam[c + (size_t)q * ldd] = (ex * (cx(q) - ux) + ey * (cy(q) - uy)) * inv;
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -123,11 +126,9 @@ __global__ void __launch_bounds__(256)
 // one stream are ordered, so the sum is race-free and deterministic).
 template <int ST>
 __global__ void __launch_bounds__(256)
-    k_adj_step(const void *fn, const typename Traits<ST>::R *__restrict__ lam_in,
-               typename Traits<ST>::R *__restrict__ lam_out, int nx, int ny, int ldd,
-               typename Traits<ST>::R omega, float inv_scale, double *partials) {
+    k_adj_step(const void *fn, const double *__restrict__ lam_in, double *__restrict__ lam_out,
+               int nx, int ny, int ldd, double omega, float inv_scale, double *partials) {
     using T = typename Traits<ST>::T;
-    using R = typename Traits<ST>::R;
     __shared__ double sh[256];
     double acc = 0.0;
     const size_t n = (size_t)nx * ny;
@@ -140,32 +141,32 @@ __global__ void __launch_bounds__(256)
         const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
         const size_t c0 = (size_t)x * cs, cm = (size_t)xm * cs, cp = (size_t)xp * cs;
         // g^n = S f^n at cell (x, y): the forward's pull pattern
-        R g[9];
-        g[0] = load_pop<ST>(s + c0 + 0 * L + y, inv_scale);
-        g[1] = load_pop<ST>(s + cm + 1 * L + y, inv_scale);
-        g[2] = load_pop<ST>(s + c0 + 2 * L + ym, inv_scale);
-        g[3] = load_pop<ST>(s + cp + 3 * L + y, inv_scale);
-        g[4] = load_pop<ST>(s + c0 + 4 * L + yp, inv_scale);
-        g[5] = load_pop<ST>(s + cm + 5 * L + ym, inv_scale);
-        g[6] = load_pop<ST>(s + cp + 6 * L + ym, inv_scale);
-        g[7] = load_pop<ST>(s + cp + 7 * L + yp, inv_scale);
-        g[8] = load_pop<ST>(s + cm + 8 * L + yp, inv_scale);
-        R rho, ux, uy, drho;
-        cell_state<ST>(g, rho, ux, uy, drho);
-        R lam[9], mu[9];
+        double g[9];
+        g[0] = double(load_pop<ST>(s + c0 + 0 * L + y, inv_scale));
+        g[1] = double(load_pop<ST>(s + cm + 1 * L + y, inv_scale));
+        g[2] = double(load_pop<ST>(s + c0 + 2 * L + ym, inv_scale));
+        g[3] = double(load_pop<ST>(s + cp + 3 * L + y, inv_scale));
+        g[4] = double(load_pop<ST>(s + c0 + 4 * L + yp, inv_scale));
+        g[5] = double(load_pop<ST>(s + cm + 5 * L + ym, inv_scale));
+        g[6] = double(load_pop<ST>(s + cp + 6 * L + ym, inv_scale));
+        g[7] = double(load_pop<ST>(s + cp + 7 * L + yp, inv_scale));
+        g[8] = double(load_pop<ST>(s + cm + 8 * L + yp, inv_scale));
+        double rho, ux, uy, drho;
+        cell_state<Traits<ST>::shifted>(g, rho, ux, uy, drho);
+        double lam[9], mu[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) lam[q] = lam_in[c0 + (size_t)q * L + y];
         // omega sensitivity: sum_q lambda_q (feq_q - g_q), in deviation form
-        R geq[9];
-        equilibrium_shifted<R>(drho, ux, uy, geq);
-        R d = 0;
+        double geq[9];
+        equilibrium_shifted<double>(drho, ux, uy, geq);
+        double d = 0;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            const R gdev = Traits<ST>::shifted ? g[q] : g[q] - R(wq(q));
+            const double gdev = Traits<ST>::shifted ? g[q] : g[q] - wq(q);
             d += lam[q] * (geq[q] - gdev);
         }
-        acc += double(d);
-        bgk_jacobian_t<R>(rho, ux, uy, lam, omega, mu);
+        acc += d;
+        bgk_jacobian_t<double>(rho, ux, uy, lam, omega, mu);
         // S^T: lambda^n_q(x - c_q, y - cy_q) = mu_q(x, y)
         lam_out[c0 + 0 * L + y] = mu[0];
         lam_out[cm + 1 * L + y] = mu[1];
@@ -187,9 +188,8 @@ __global__ void __launch_bounds__(256)
 }
 
 // Gradient with respect to the initial macroscopic fields: f^0 = feq(rho0, u0).
-template <class R>
 __global__ void __launch_bounds__(256)
-    k_adj_initial(const R *lam0, int nx, int ny, int ldd, const double *rho0, const double *ux0,
+    k_adj_initial(const double *lam0, int nx, int ny, int ldd, const double *rho0, const double *ux0,
                   const double *uy0, double *grho, double *gux, double *guy) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
     if (y >= ny) return;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256)
     double gr = 0, gx = 0, gy = 0;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        const double l = double(lam0[c + (size_t)q * ldd]) * wq(q);
+        const double l = lam0[c + (size_t)q * ldd] * wq(q);
         const double cu = cx(q) * ux + cy(q) * uy;
         gr += l * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
         const double t = 3.0 + 9.0 * cu;
@@ -288,7 +288,7 @@ int plbm_adjoint_create(int nx, int ny, int storage, int nsteps, int checkpoint_
     }
     a->stream = a->lat->stream;
     a->buf_bytes = a->lat->elems * storage_bytes(storage);
-    a->lam_bytes = a->lat->elems * (storage == PLBM_F32S ? 4 : 8);
+    a->lam_bytes = a->lat->elems * sizeof(double);  // FP64 adjoint state for every storage
     auto fail = [&](const char *m) {
         (void)hipGetLastError();
         set_error(m);
@@ -359,10 +359,9 @@ int plbm_adjoint_forward_dev(plbm_adjoint *a, const double *rho0, const double *
             set_error("adjoint: FP16 storage is not supported");
             return PLBM_ERR_ARG;
         } else {
-            using R = typename Traits<SS>::R;
             hipLaunchKernelGGL(k_adj_final<SS>, dim3(nb), dim3(256), 0, a->stream, h->f[cur], a->nx,
                                a->ny, h->ldd, float(1.0 / h->scale), uxt, uyt,
-                               static_cast<R *>(a->lam[0]), a->d_partials);
+                               static_cast<double *>(a->lam[0]), a->d_partials);
             PLBM_HIP(hipGetLastError());
             return PLBM_OK;
         }
@@ -398,11 +397,10 @@ int plbm_adjoint_backward_dev(plbm_adjoint *a, double *grho0, double *gux0, doub
                 if constexpr (SS == PLBM_F16S) {
                     return PLBM_ERR_ARG;
                 } else {
-                    using R = typename Traits<SS>::R;
                     hipLaunchKernelGGL(k_adj_step<SS>, dim3(nb), dim3(256), 0, a->stream,
-                                       seg(a, n - n0), static_cast<const R *>(a->lam[cur]),
-                                       static_cast<R *>(a->lam[cur ^ 1]), a->nx, a->ny, h->ldd,
-                                       R(a->omega), float(1.0 / h->scale), a->d_partials);
+                                       seg(a, n - n0), static_cast<const double *>(a->lam[cur]),
+                                       static_cast<double *>(a->lam[cur ^ 1]), a->nx, a->ny,
+                                       h->ldd, a->omega, float(1.0 / h->scale), a->d_partials);
                     PLBM_HIP(hipGetLastError());
                     return PLBM_OK;
                 }
@@ -411,19 +409,10 @@ int plbm_adjoint_backward_dev(plbm_adjoint *a, double *grho0, double *gux0, doub
         }
     }
     // lam[cur] = lambda^0 -> gradients of the initial fields
-    PLBM_TRY(dispatch_storage(a->storage, [&](auto S) {
-        constexpr int SS = decltype(S)::value;
-        if constexpr (SS == PLBM_F16S) {
-            return PLBM_ERR_ARG;
-        } else {
-            using R = typename Traits<SS>::R;
-            hipLaunchKernelGGL(k_adj_initial<R>, dim3((a->ny + 255) / 256, a->nx), dim3(256), 0,
-                               a->stream, static_cast<const R *>(a->lam[cur]), a->nx, a->ny,
-                               h->ldd, a->rho0, a->ux0, a->uy0, grho0, gux0, guy0);
-            PLBM_HIP(hipGetLastError());
-            return PLBM_OK;
-        }
-    }));
+    hipLaunchKernelGGL(k_adj_initial, dim3((a->ny + 255) / 256, a->nx), dim3(256), 0, a->stream,
+                       static_cast<const double *>(a->lam[cur]), a->nx, a->ny, h->ldd, a->rho0,
+                       a->ux0, a->uy0, grho0, gux0, guy0);
+    PLBM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, a->stream, a->d_partials, nb,
                        a->d_scalar + 1);
     PLBM_HIP(hipGetLastError());

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/plbm.h"
 
@@ -96,6 +97,9 @@ struct StepArgs {
     double omega, lambda_odd;
     float omega_f, lambda_odd_f;
     float scale, inv_scale; // FP16 storage scale (1 otherwise)
+    // Pass kernels only: when non-null, the pass also stores the state one step before
+    // its last (level NS-1, post-collision) here, in the layout of dst.
+    void *lag = nullptr;
 };
 
 // Strip length (columns per thread) of the stencil kernels that walk along x with a
@@ -142,6 +146,23 @@ constexpr int kReduceBlocks = 1024;
 
 }  // namespace plbm
 
+struct plbm_lattice;
+
+namespace plbm {
+// One temporally blocked pass of ns (2..8) steps src -> dst; with lag non-null it also
+// stores the state one step before the pass's last there (multistep.hip).
+int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
+                 void *lag = nullptr);
+// Run bookkeeping shared by the pass runs (multistep.hip): pass depths of a run, the
+// lattice's third buffer (nullptr when it cannot be allocated), and the final assignment
+// of device storage to slots.
+std::vector<int> pass_plan(int nsteps, int ns);
+void *pass_aux(plbm_lattice *h);
+void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void *lag,
+                void *spare, int *iold, int *inew);
+
+}  // namespace plbm
+
 struct plbm_lattice {
     int nx = 0, ny = 0, ldd = 0, nf = 0, storage = 0, device = 0;
     hipStream_t stream = nullptr;  // the stream work is issued on
@@ -156,6 +177,11 @@ struct plbm_lattice {
     int dugks_ns = 0;                    // DUGKS steps per pass (0 = chosen from the grid)
     int fd_ns = 0;                       // FV/FD operator steps per pass of plbm_run_op
     int pass_segs = 0;                   // column segments per row band of a pass (0 = auto)
+    // third population buffer of the pass runs (allocated on the first run in passes):
+    // the last pass of a run writes the state one step before the latest into it
+    void *aux = nullptr;
+    bool aux_failed = false;             // its allocation failed once; runs end on a single step
+    bool aux_enabled = true;             // plbm_set_pass_lag_buffer
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

@@ -319,7 +319,6 @@ int launch_step_cfg(plbm_lattice *h, int model, const void *src, void *dst, Laun
     });
 }
 
-int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst);
 int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew);
 int default_steps_per_launch(const plbm_lattice *h);
 
@@ -520,6 +519,7 @@ int plbm_destroy(plbm_lattice *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (int b = 0; b < kMaxBuffers; ++b)
         if (h->f[b]) (void)hipFree(h->f[b]);
+    if (h->aux) (void)hipFree(h->aux);
     if (h->d_tmp) (void)hipFree(h->d_tmp);
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->d_red) (void)hipFree(h->d_red);

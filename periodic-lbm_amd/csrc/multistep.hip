@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "multistep.hpp"
 
@@ -77,10 +78,12 @@ int max_steps_per_pass(int) { return kMaxPassSteps; }
 // each wavefront's sequential walk.
 int default_pass_segments(int nx) { return std::max(1, std::min(64, nx / 8)); }
 
-int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst) {
+int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
+                 void *lag) {
     StepArgs a{};
     a.src = src;
     a.dst = dst;
+    a.lag = lag;
     a.nx = h->nx;
     a.ny = h->ny;
     a.ldd = h->ldd;
@@ -129,13 +132,57 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst)
     });
 }
 
+// Depths of the passes of a run of nsteps >= 2 steps with passes of at most ns >= 2 steps:
+// the fewest passes, their depths differing by at most one, the deepest last (so the last
+// pass, which also stores the state one step before the latest, has at least 2 steps).
+// A pass of depth d costs about one HBM sweep for every d up to 6 at 8192^2 (TRT FP64,
+// us per launch for d = 2..8: 1969 2004 1995 1952 1992 2249 2447), so splitting 20 steps
+// as 6 + 7 + 7 costs less than 8 + 8 + 4 (6490 against 6889 us) and much less than
+// 8 + 8 + 3 followed by a single-step sweep (8587 us).
+std::vector<int> pass_plan(int nsteps, int ns) {
+    const int passes = (nsteps + ns - 1) / ns;
+    std::vector<int> d((size_t)passes, nsteps / passes);
+    for (int p = 0; p < nsteps % passes; ++p) d[(size_t)(passes - 1 - p)] += 1;
+    return d;
+}
+
+// The third buffer that the last pass of a run writes the lagged state into.  Allocated
+// on the first run in passes; when that fails (a grid that fits two buffers but not
+// three), or the caller has switched it off (plbm_set_pass_lag_buffer), runs end on a
+// single-step launch instead, as the state-lag rule needs.
+void *pass_aux(plbm_lattice *h) {
+    if (!h->aux_enabled) return nullptr;
+    if (h->aux || h->aux_failed) return h->aux;
+    if (hipMalloc(&h->aux, h->elems * storage_bytes(h->storage)) != hipSuccess) {
+        (void)hipGetLastError();
+        h->aux = nullptr;
+        h->aux_failed = true;
+    }
+    return h->aux;
+}
+
+// Ends a run whose latest state is at `latest` and whose state one step earlier is at
+// `lag` (`spare` is the third buffer, contents no longer needed): slot ids follow the
+// reference's swap rule (after nsteps swaps the latest state is in slot b0 when nsteps is
+// odd), and the device storage of slots a0 / b0 and of the spare buffer is permuted to
+// match.
+void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void *lag,
+                void *spare, int *iold, int *inew) {
+    const int want = (nsteps % 2) ? b0 : a0;
+    const int other = want == a0 ? b0 : a0;
+    h->f[want] = latest;
+    h->f[other] = lag;
+    if (spare != nullptr) h->aux = spare;
+    *iold = want;
+    *inew = other;
+}
+
 // nsteps steps of `model` with the reference's bookkeeping (each step reads *iold, writes
-// *inew, swaps them).  With ns > 1 all but the last step run in passes of up to ns steps;
-// the last step is a single-step launch, so that the other buffer ends up holding the
-// state one step before the latest (what update_macros reads, as in the reference).
-// A pass moves the latest state to the other buffer whatever its step count, so when the
-// parity of passes differs from the parity of steps the two slots exchange their device
-// storage at the end.
+// *inew, swaps them).  With ns > 1 the steps run in passes of up to ns steps (pass_plan);
+// the last pass also stores the state one step before the latest into the third buffer,
+// which becomes the other slot's storage, as update_macros expects of it (the reference's
+// inew after perform_lbm_step).  Without a third buffer the last step is a single-step
+// launch instead.
 int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew) {
     if (ns <= 1 || nsteps < 2) {
         for (int s = 0; s < nsteps; ++s) {
@@ -146,21 +193,25 @@ int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *in
     }
     const int a0 = *iold, b0 = *inew;
     void *latest = h->f[a0], *other = h->f[b0];
-    const int body = nsteps - 1;
-    for (int done = 0; done < body;) {
-        const int k = std::min(ns, body - done);
-        if (k == 1) PLBM_TRY(launch_step(h, model, latest, other));
-        else PLBM_TRY(launch_multi(h, model, k, latest, other));
-        std::swap(latest, other);
-        done += k;
+    void *aux = pass_aux(h);
+    if (aux == nullptr) {
+        for (int k : pass_plan(nsteps - 1, ns)) {
+            if (k == 1) PLBM_TRY(launch_step(h, model, latest, other));
+            else PLBM_TRY(launch_multi(h, model, k, latest, other));
+            std::swap(latest, other);
+        }
+        PLBM_TRY(launch_step(h, model, latest, other));
+        finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
+        return PLBM_OK;
     }
-    PLBM_TRY(launch_step(h, model, latest, other));
-    std::swap(latest, other);
-    // reference parity: after nsteps swaps the latest state is in slot b0 when nsteps is odd
-    const int want = (nsteps % 2) ? b0 : a0;
-    if (h->f[want] != latest) std::swap(h->f[a0], h->f[b0]);
-    *iold = want;
-    *inew = want == a0 ? b0 : a0;
+    const std::vector<int> plan = pass_plan(nsteps, ns);
+    for (size_t p = 0; p + 1 < plan.size(); ++p) {
+        if (plan[p] == 1) PLBM_TRY(launch_step(h, model, latest, other));
+        else PLBM_TRY(launch_multi(h, model, plan[p], latest, other));
+        std::swap(latest, other);
+    }
+    PLBM_TRY(launch_multi(h, model, plan.back(), latest, other, aux));
+    finish_run(h, a0, b0, nsteps, other, aux, latest, iold, inew);
     return PLBM_OK;
 }
 
@@ -202,6 +253,19 @@ extern "C" int plbm_set_pass_segments(plbm_lattice *h, int segments) {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(segments >= 0, PLBM_ERR_ARG, "segments must be >= 0 (0 = automatic)");
     h->pass_segs = segments;
+    return PLBM_OK;
+}
+
+extern "C" int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(enable == 0 || enable == 1, PLBM_ERR_ARG, "enable must be 0 or 1");
+    h->aux_enabled = enable != 0;
+    if (!h->aux_enabled && h->aux) {
+        PLBM_HIP(hipStreamSynchronize(h->stream));
+        PLBM_HIP(hipFree(h->aux));
+        h->aux = nullptr;
+    }
+    if (h->aux_enabled) h->aux_failed = false;
     return PLBM_OK;
 }
 

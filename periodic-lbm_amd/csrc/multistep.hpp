@@ -65,7 +65,11 @@ constexpr int kStageSlots = 10;
 // rounded down to one of the counts the staged pass produces (rounding down is a stronger
 // wait): vmcnt takes an immediate.
 __device__ __forceinline__ void wait_vm_at_most(int n) {
-    if (n >= 36) asm volatile("s_waitcnt vmcnt(36)" ::: "memory");
+    if (n >= 54) asm volatile("s_waitcnt vmcnt(54)" ::: "memory");
+    else if (n >= 45) asm volatile("s_waitcnt vmcnt(45)" ::: "memory");
+    else if (n >= 41) asm volatile("s_waitcnt vmcnt(41)" ::: "memory");
+    else if (n >= 36) asm volatile("s_waitcnt vmcnt(36)" ::: "memory");
+    else if (n >= 32) asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
     else if (n >= 27) asm volatile("s_waitcnt vmcnt(27)" ::: "memory");
     else if (n >= 23) asm volatile("s_waitcnt vmcnt(23)" ::: "memory");
     else if (n >= 18) asm volatile("s_waitcnt vmcnt(18)" ::: "memory");
@@ -141,23 +145,29 @@ struct ColumnStage {
 
     // Wait until column i of a walk of `iters` columns is in LDS.  The prologue issued
     // the loads of columns 0 .. kStageAhead-1; iteration t issues those of column
-    // t + kStageAhead (if it exists) and then 9 stores (if t >= first_store).  Vector
-    // memory operations complete in issue order, so it suffices that no more remain
-    // outstanding than were issued after the loads of column i.  Also waits for the LDS
-    // reads of the previous column, whose buffer the next loads overwrite.
-    __device__ __forceinline__ void wait(int i, int iters, int first_store) const {
+    // t + kStageAhead (if it exists) and then its stores: 9 of the last level (if
+    // t >= first_store) and, when the pass also writes the level before the last (`lag`),
+    // 9 of that level (if first_store - 1 <= t < iters - 1).  Vector memory operations
+    // complete in issue order, so it suffices that no more remain outstanding than were
+    // issued after the loads of column i.  Also waits for the LDS reads of the previous
+    // column, whose buffer the next loads overwrite.
+    __device__ __forceinline__ void wait(int i, int iters, int first_store, bool lag) const {
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        auto stores = [&](int t) {
+            return (t >= first_store ? 9 : 0) +
+                   (lag && t >= first_store - 1 && t < iters - 1 ? 9 : 0);
+        };
         int after = 0, t0;
         if (i < kStageAhead) {
             for (int j = i + 1; j < kStageAhead; ++j) after += j < iters ? loads : 0;
             t0 = 0;
         } else {
-            after += i - kStageAhead >= first_store ? 9 : 0;
+            after += stores(i - kStageAhead);
             t0 = i - kStageAhead + 1;
         }
         for (int t = t0; t < i; ++t) {
             after += t + kStageAhead < iters ? loads : 0;
-            after += t >= first_store ? 9 : 0;
+            after += stores(t);
         }
         wait_vm_at_most(after);
     }
@@ -201,7 +211,8 @@ __device__ __forceinline__ R pull_y(int q, R v) {
 
 // The calling wavefront advances rows [VR*wy, VR*wy + VR) (clipped to the grid) x columns
 // [x0, x0 + ncols) (0 <= x0, x0 + ncols <= nx) of the grid described by `a` NS steps,
-// a.src -> a.dst.  All 64 lanes must call it together (lane shifts and shuffles).
+// a.src -> a.dst, and, when a.lag is set, stores level NS-1 of the same cells there.
+// All 64 lanes must call it together (lane shifts and shuffles).
 template <int MODEL, int ST, int NS>
 __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
     using T = typename Traits<ST>::T;
@@ -217,6 +228,7 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     const size_t L = (size_t)a.ldd, cs = a.cs;
     const T *__restrict__ s = static_cast<const T *>(a.src);
     T *__restrict__ d = static_cast<T *>(a.dst);
+    T *__restrict__ lag = static_cast<T *>(a.lag);  // wave-uniform
     R W[NS][3][9];  // [level][newest column, -1, -2][q]
 #pragma unroll
     for (int k = 0; k < NS; ++k)
@@ -257,7 +269,7 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
         if (j < iters) fetch(x0 - NS + j, j);
     for (int i = 0; i < iters; ++i) {
         if constexpr (kStage) {
-            st.wait(i, iters, 2 * NS);
+            st.wait(i, iters, 2 * NS, lag != nullptr);
             const double *slot = st.column(i);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
@@ -285,6 +297,13 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
                 f[q] = cx(q) > 0 ? W[k - 1][2][q] : (cx(q) == 0 ? W[k - 1][1][q] : W[k - 1][0][q]);
             collide_cell<MODEL, ST>(f, a);
             if (k < NS) {
+                // level NS-1 reaches column x0 + j at iteration 2 NS - 1 + j
+                if (k == NS - 1 && lag != nullptr && i >= 2 * NS - 1 && i < 2 * NS - 1 + ncols &&
+                    out) {
+                    const size_t co = (size_t)(x0 - 2 * NS + 1 + i) * cs;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) store_pop<ST>(lag + co + q * L + y, f[q], a.scale);
+                }
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
                     W[k][2][q] = W[k][1][q];

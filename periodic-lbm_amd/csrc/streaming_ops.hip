@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(256)
     const size_t L = (size_t)a.ldd, cs = a.cs;
     const double *__restrict__ s = static_cast<const double *>(a.src);
     double *__restrict__ d = static_cast<double *>(a.dst);
+    double *__restrict__ lag = static_cast<double *>(a.lag);  // wave-uniform
     auto column = [&](int c) {
         c %= nx;
         if (c < 0) c += nx;
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(256)
         if (j < iters) st.fetch(s, column(x0 - NS + j), L, j % kStageBufs);
     for (int i = 0; i < iters; ++i) {
         double f[9];
-        st.wait(i, iters, 2 * NS);
+        st.wait(i, iters, 2 * NS, a.lag != nullptr);
         {
             const double *slot = st.column(i);
 #pragma unroll
@@ -215,6 +216,14 @@ __global__ void __launch_bounds__(256)
                 const size_t co = (size_t)(x0 - 2 * NS + i) * cs;
 #pragma unroll
                 for (int q = 0; q < 9; ++q) __builtin_nontemporal_store(f[q], d + co + q * L + y);
+            } else if (k + 2 == NS && lag != nullptr && i >= 2 * NS - 1 &&
+                       i < 2 * NS - 1 + ncols && out) {
+                // level NS-1 (the state one step before the pass's last) of column
+                // x0 + i - (2 NS - 1)
+                const size_t co = (size_t)(x0 - 2 * NS + 1 + i) * cs;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    __builtin_nontemporal_store(f[q], lag + co + q * L + y);
             }
         }
     }
@@ -300,9 +309,10 @@ int check_op_args(plbm_lattice *h, int op, int src, int dst, double dt) {
 }
 
 int launch_fd_multi(plbm_lattice *h, int op, int model, int ns, const void *src, void *dst,
-                    double dt) {
+                    double dt, void *lag = nullptr) {
     const double p2s = 0.5 / std::sqrt(2.0);  // src/fvm_bardow.F90:712
-    const StepArgs a = fd_args(h, src, dst);
+    StepArgs a = fd_args(h, src, dst);
+    a.lag = lag;
     const int waves_y = (h->ny + 55) / 56;
     const int nw = waves_y * ((h->nx + kMultiXS - 1) / kMultiXS);
     const dim3 grid((unsigned)((nw + 3) / 4));
@@ -391,23 +401,28 @@ extern "C" int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *
         }
         return PLBM_OK;
     }
-    // all steps but the last in passes; the last a single fused step, so that the other
-    // slot ends with the state one step earlier (as after perform_step)
+    // passes of up to ns steps, evenly split, the deepest last; the last pass also stores
+    // the state one step before the latest into the third buffer, which becomes the other
+    // slot's storage (as after perform_step).  Without a third buffer the last step is a
+    // single fused step instead.
     const int a0 = *iold, b0 = *inew;
     void *latest = h->f[a0], *other = h->f[b0];
-    for (int done = 0; done < nsteps - 1;) {
-        const int k = std::min(ns, nsteps - 1 - done);
-        if (k == 1) PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
-        else PLBM_TRY(launch_fd_multi(h, op, model, k, latest, other, dt));
+    void *aux = pass_aux(h);
+    const std::vector<int> plan = pass_plan(aux ? nsteps : nsteps - 1, ns);
+    for (size_t p = 0; p + 1 < plan.size(); ++p) {
+        if (plan[p] == 1) PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+        else PLBM_TRY(launch_fd_multi(h, op, model, plan[p], latest, other, dt));
         std::swap(latest, other);
-        done += k;
     }
-    PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+    if (aux) {
+        PLBM_TRY(launch_fd_multi(h, op, model, plan.back(), latest, other, dt, aux));
+        finish_run(h, a0, b0, nsteps, other, aux, latest, iold, inew);
+        return PLBM_OK;
+    }
+    if (plan.back() == 1) PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+    else PLBM_TRY(launch_fd_multi(h, op, model, plan.back(), latest, other, dt));
     std::swap(latest, other);
-    // reference parity: after nsteps swaps the latest state is in slot b0 when nsteps is odd
-    const int want = (nsteps % 2) ? b0 : a0;
-    if (h->f[want] != latest) std::swap(h->f[a0], h->f[b0]);
-    *iold = want;
-    *inew = want == a0 ? b0 : a0;
+    PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
+    finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
     return PLBM_OK;
 }

@@ -118,6 +118,12 @@ class Lattice:
         check(lib().plbm_get_pass_segments(self.h, ctypes.byref(v)))
         return v.value
 
+    def set_pass_lag_buffer(self, enable: bool):
+        """Third buffer of runs in passes (default on): the last pass also writes the
+        state one step before the latest; off, runs end on a single step instead.  Results
+        are identical either way."""
+        check(lib().plbm_set_pass_lag_buffer(self.h, int(bool(enable))))
+
     def autotune(self, model: str | int, buf: int = 0, reps: int = 10) -> float:
         """Time every launch shape of the fused step and the temporal-blocking depths; keep
         the fastest.  Returns ms per lattice-update step."""

@@ -18,9 +18,15 @@ data files in this directory:
   ref_dugks_app.json  sha256 of the raw fields tests/fortran/dugks_check.f90 writes
   ref_split.npz   the reference built with -DSPLIT (oracle/_ref/split): collide_bgk
                   (bgk_kernel_cache) and collide_trt (trt_split), 1/10/100 steps
+  ref_shear.npz   shear_layer_t's grid evaluation (src/benchmarks/shear_layer.f90:25-52)
+                  with the batched sweep's parameters (L = N, k = 80, delta = 0.05,
+                  U0 = 0.04) at N = 32, 96, 512 and on a non-square grid with L != N:
+                  full fields up to 96^2, the x- and y-profiles and sha256 at 512^2
+  ref_triple.npz  perform_triple_step (src/fvm_bardow.F90:322-340): all three buffers and
+                  the ids after 1 and 5 steps, TRT and BGK, 16 x 12 (nf = 3)
 
 Usage (in a container that has the reference tree and flang):
-  make -C oracle ref && python tests/golden/make_golden.py [--app]
+  make -C oracle ref && python tests/golden/make_golden.py [--app | --pins-only]
 """
 from __future__ import annotations
 
@@ -131,6 +137,48 @@ def params(R):
         out["equilibrium"].append([rho, ux, uy, R.equilibrium(rho, ux, uy).tolist()])
     g.close()
     (HERE / "ref_params.json").write_text(json.dumps(out, indent=1))
+
+
+def shear(R):
+    out = {}
+    cases = [(32, 32, 32.0), (96, 96, 96.0), (512, 512, 512.0), (48, 40, 64.0)]
+    u0, k, delta = 0.04, 80.0, 0.05
+    for nx, ny, length in cases:
+        rho, ux, uy = R.shear(nx, ny, length, u0, k, delta)
+        key = f"{nx}x{ny}"
+        out[f"{key}_params"] = np.array([length, u0, k, delta])
+        if nx * ny <= 96 * 96:
+            out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+        else:
+            # ux depends on y only and uy on x only: the profiles plus a fingerprint
+            out[f"{key}_ux_col"], out[f"{key}_uy_row"] = ux[:, 0].copy(), uy[0, :].copy()
+            out[f"{key}_sha256"] = np.array([sha(rho), sha(ux), sha(uy)])
+    np.savez_compressed(HERE / "ref_shear.npz", **out)
+
+
+def triple(R):
+    out = {}
+    nx, ny = 16, 12
+    rng = np.random.default_rng(2024)
+    rho = np.asfortranarray(1.0 + 0.01 * rng.standard_normal((ny, nx)))
+    ux = np.asfortranarray(0.04 * rng.standard_normal((ny, nx)))
+    uy = np.asfortranarray(0.04 * rng.standard_normal((ny, nx)))
+    out["rho"], out["ux"], out["uy"], out["params"] = rho, ux, uy, np.array([0.01, 1.0, 0.25])
+    for m in ("trt", "bgk"):
+        g = R.grid(nx, ny, m, nf=3)  # all buffers zero before the first step
+        g.set_properties(0.01, 1.0, 0.25)
+        g.set_macros(rho, ux, uy)
+        g.set_pdf_to_equilibrium()
+        done = 0
+        for n in (1, 5):
+            g.triple_step(n - done)
+            done = n
+            iold, inew, imid, _ = g.indices()
+            out[f"{m}_{n}_ids"] = np.array([iold, inew, imid])
+            for b in (1, 2, 3):
+                out[f"{m}_{n}_buf{b}"] = g.get_f(b)[:ny]
+        g.close()
+    np.savez_compressed(HERE / "ref_triple.npz", **out)
 
 
 def app():
@@ -277,7 +325,13 @@ if __name__ == "__main__":
         streaming_ops(R)
         vortex_app()
         sys.exit(0)
+    if "--pins-only" in sys.argv:
+        shear(R)
+        triple(R)
+        sys.exit(0)
     if "--outputs-only" not in sys.argv:
+        shear(R)
+        triple(R)
         params(R)
         small(R)
         config1(R)

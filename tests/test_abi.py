@@ -57,6 +57,33 @@ def test_taylor_green_field_equals_oracle(P, oracle, n, u0, nu, t):
         np.testing.assert_array_equal(a, b)
 
 
+def test_shear_layer_field_equals_reference(P):
+    """The engine's shear-layer field (config 4's initial state) against the reference's
+    own evaluation (ref_shear.npz, src/benchmarks/shear_layer.f90:25-52), bit for bit."""
+    import hashlib
+    G = np.load(ROOT / "tests" / "golden" / "ref_shear.npz")
+    sha = lambda a: hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+    for key in ("32x32", "96x96", "512x512", "48x40"):
+        nx, ny = map(int, key.split("x"))
+        length, u0, k, delta = G[f"{key}_params"]
+        rho, ux, uy = P.ShearLayer(length, u0, k, delta).eval(nx, ny)
+        if f"{key}_ux" in G:
+            for a, name in ((rho, "rho"), (ux, "ux"), (uy, "uy")):
+                np.testing.assert_array_equal(a, G[f"{key}_{name}"])
+        else:
+            assert [sha(rho), sha(ux), sha(uy)] == G[f"{key}_sha256"].tolist()
+
+
+def test_noise_published_splitmix64_vector(P):
+    """plbm_noise(seed, c) = output c + 1 of SplitMix64 from state `seed`, top 53 bits:
+    the published outputs from state 1234567 (Rosetta Code, Pseudo-random numbers/
+    Splitmix64)."""
+    outs = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+            4593380528125082431, 16408922859458223821]
+    for c, z in enumerate(outs):
+        assert P.noise(1234567, c) == (z >> 11) * 2.0 ** -53
+
+
 def test_shear_layer_field_equals_oracle(P, oracle):
     for n in (32, 96, 512):
         a = P.ShearLayer(float(n), 0.04, 80.0, 0.05).eval(n, n)

@@ -81,20 +81,58 @@ def test_passes_match_single_steps_shifted(P, gpu, storage, model):
         np.testing.assert_allclose(r, r0, rtol=0, atol=tol)
 
 
-@pytest.mark.parametrize("nsteps", [1, 2, 3, 4, 5, 6, 9])
-def test_step_counts_and_slots(P, gpu, nsteps):
-    """Every step count (single passes, partial last pass, exact multiples) keeps the
-    reference's slot contents; the slots' device storage may be exchanged."""
+@pytest.mark.parametrize("lag_buffer", [True, False])
+@pytest.mark.parametrize("ns", [2, 3, 4, 8])
+def test_step_counts_and_slots(P, gpu, ns, lag_buffer):
+    """Every step count 1..20 (one pass, several, uneven splits, a single step) keeps the
+    reference's slot contents, with the last pass writing the lagged state into the third
+    buffer or (lag buffer off) a final single step; the slots' device storage may move.
+    A second run continues from the returned ids."""
     nx, ny = 96, 80
-    ref, ri, rn = run(P, "f64", "trt", 1, nx, ny, nsteps)
-    lat, i, n = run(P, "f64", "trt", 4, nx, ny, nsteps)
-    assert (i, n) == (ri, rn) == ((1, 0) if nsteps % 2 else (0, 1))
-    np.testing.assert_array_equal(lat.download_f(i), ref.download_f(ri))
-    np.testing.assert_array_equal(lat.download_f(n), ref.download_f(rn))
-    # a second run continues from the returned ids
-    i2, n2 = lat.run("trt", 5, i, n)
-    r2, q2 = ref.run("trt", 5, ri, rn)
-    np.testing.assert_array_equal(lat.download_f(i2), ref.download_f(r2))
+    print(f"case step counts ns={ns} lag_buffer={lag_buffer}", flush=True)
+    for nsteps in range(1, 21):
+        ref, ri, rn = run(P, "f64", "trt", 1, nx, ny, nsteps)
+        rho, ux, uy = fields(nx, ny, 1)
+        lat = P.Lattice(nx, ny, 2, "f64")
+        lat.set_params(1.0 / (3 * 0.004 + 0.5), 0.25)
+        lat.set_steps_per_launch("trt", ns)
+        lat.set_pass_lag_buffer(lag_buffer)
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run("trt", nsteps, 0, 1)
+        assert (i, n) == (ri, rn) == ((1, 0) if nsteps % 2 else (0, 1)), nsteps
+        np.testing.assert_array_equal(lat.download_f(i), ref.download_f(ri), err_msg=str(nsteps))
+        np.testing.assert_array_equal(lat.download_f(n), ref.download_f(rn), err_msg=str(nsteps))
+        i2, n2 = lat.run("trt", 7, i, n)
+        r2, q2 = ref.run("trt", 7, ri, rn)
+        assert (i2, n2) == (r2, q2)
+        np.testing.assert_array_equal(lat.download_f(i2), ref.download_f(r2))
+        np.testing.assert_array_equal(lat.download_f(n2), ref.download_f(q2))
+        lat.close()
+        ref.close()
+
+
+@pytest.mark.parametrize("storage", ["f32s", "f16s"])
+def test_lag_slot_shifted_storage(P, gpu, storage):
+    """4- and 2-byte storage: the lagged slot written by the last pass holds the state one
+    step before the latest (as the run with the lag buffer off, which ends on a single
+    step, to the storage's roundoff)."""
+    nx, ny, nsteps = 150, 100, 13
+    out = []
+    for lag_buffer in (True, False):
+        rho, ux, uy = fields(nx, ny, 4)
+        lat = P.Lattice(nx, ny, 2, storage)
+        lat.set_params(1.0 / (3 * 0.004 + 0.5), 0.25)
+        lat.set_steps_per_launch("trt", 8)
+        lat.set_pass_lag_buffer(lag_buffer)
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run("trt", nsteps, 0, 1)
+        out.append(((i, n), lat.macros(i), lat.macros(n)))
+    assert out[0][0] == out[1][0]
+    tol = SHIFTED_TOL[storage]
+    for a, b in zip(out[0][1] + out[0][2], out[1][1] + out[1][2]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=tol)
+    # the two slots differ by one step
+    assert np.abs(out[0][1][1] - out[0][2][1]).max() > 10 * tol
 
 
 def test_full_size_passes_bit_exact_4096(P, gpu, oracle):

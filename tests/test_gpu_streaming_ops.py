@@ -182,6 +182,31 @@ def test_operator_passes_equal_single_steps(P, gpu, op, model, per_pass, nx, ny,
     np.testing.assert_array_equal(b1, b2)
 
 
+@pytest.mark.parametrize("lag_buffer", [True, False])
+@pytest.mark.parametrize("per_pass", [2, 3])
+def test_operator_pass_step_counts(P, gpu, per_pass, lag_buffer):
+    """plbm_run_op for every step count 1..12, passes of 2 / 3 with the lagged state from
+    the last pass (or a final single step), against single fused steps: both slots."""
+    nx, ny = 96, 70
+    print(f"case fd step counts per_pass={per_pass} lag_buffer={lag_buffer}", flush=True)
+    for nsteps in range(1, 13):
+        rho, ux, uy = fields(nx, ny, 3)
+        out = []
+        for pp in (1, per_pass):
+            lat = P.Lattice(nx, ny, 2, "f64")
+            lat.set_params(1.0 / (3 * 0.01 + 0.5), 0.25)
+            lat.set_op_steps_per_pass(pp)
+            lat.set_pass_lag_buffer(lag_buffer)
+            lat.equilibrium_init(0, rho, ux, uy)
+            i, n = lat.run_op("fdm_sofonea", "trt", nsteps, 0, 1, 1.0)
+            out.append((i, n, lat.download_f(i), lat.download_f(n)))
+            lat.close()
+        (i1, n1, a1, b1), (i2, n2, a2, b2) = out
+        assert (i1, n1) == (i2, n2), nsteps
+        np.testing.assert_array_equal(a1, a2, err_msg=str(nsteps))
+        np.testing.assert_array_equal(b1, b2, err_msg=str(nsteps))
+
+
 def test_operator_passes_full_size(P, gpu, oracle):
     """4096^2, stream_fvm_bardow + BGK, 5 steps in passes (automatic depth) vs the oracle."""
     n, nsteps, dt = 4096, 5, 1.0

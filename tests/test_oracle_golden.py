@@ -160,3 +160,71 @@ def test_split_build_bitwise(oracle, nx, ny):
     latest, _ = oracle.run("bgk", nx, ny, fa, fb, 100, omega, 0.25)
     d = np.abs(latest[:ny] - G[f"{key}_bgk_100_latest"]).max()
     assert 0 < d < 1e-13
+
+
+def test_shear_layer_bitwise():
+    """orc_shear_eval against shear_layer_t's grid evaluation run by the reference
+    (src/benchmarks/shear_layer.f90:25-52; ref_shear.npz), the batched sweep's parameters
+    at N = 32, 96, 512 and a non-square grid with L != N."""
+    G = np.load(GOLDEN / "ref_shear.npz")
+    O = po.Oracle()
+    for key in ("32x32", "96x96", "512x512", "48x40"):
+        nx, ny = map(int, key.split("x"))
+        length, u0, k, delta = G[f"{key}_params"]
+        rho, ux, uy = O.shear(nx, ny, length, u0, k, delta)
+        if f"{key}_ux" in G:
+            np.testing.assert_array_equal(rho, G[f"{key}_rho"])
+            np.testing.assert_array_equal(ux, G[f"{key}_ux"])
+            np.testing.assert_array_equal(uy, G[f"{key}_uy"])
+        else:
+            np.testing.assert_array_equal(ux[:, 0], G[f"{key}_ux_col"])
+            np.testing.assert_array_equal(uy[0, :], G[f"{key}_uy_row"])
+            assert [sha(rho), sha(ux), sha(uy)] == G[f"{key}_sha256"].tolist()
+
+
+@pytest.mark.parametrize("model", ["trt", "bgk"])
+def test_triple_step_bitwise(oracle, model):
+    """perform_triple_step (src/fvm_bardow.F90:322-340) composed from the oracle's stream,
+    copy and collision, against the reference's three buffers and ids after 1 and 5 steps
+    (ref_triple.npz): the third slot holds the pre-collision (streamed) state of the last
+    step, the other the previous step's."""
+    G = np.load(GOLDEN / "ref_triple.npz")
+    nx, ny = 16, 12
+    nu, dt, magic = G["params"]
+    par = oracle.set_properties(nu, dt, magic)
+    bufs = {b: po.new_f(nx, ny) for b in (1, 2, 3)}
+    iold, inew, imid = 2, 1, 3  # alloc_grid (src/fvm_bardow.F90:175-180)
+    oracle.set_pdf_to_equilibrium(nx, ny, np.asfortranarray(G["rho"]), np.asfortranarray(G["ux"]),
+                                  np.asfortranarray(G["uy"]), bufs[iold])
+    done = 0
+    for n in (1, 5):
+        for _ in range(n - done):
+            oracle.stream(nx, ny, bufs[iold], bufs[inew])
+            bufs[iold][...] = bufs[inew]
+            oracle.collide(model, nx, ny, bufs[inew], par["omega"], par["trt_magic"])
+            iold, inew, imid = inew, imid, iold
+        done = n
+        assert [iold, inew, imid] == G[f"{model}_{n}_ids"].tolist()
+        for b in (1, 2, 3):
+            np.testing.assert_array_equal(bufs[b][:ny], G[f"{model}_{n}_buf{b}"], err_msg=f"{n} {b}")
+    # a post-collision copy in the third slot would not match
+    assert not np.array_equal(G[f"{model}_5_buf{imid}"], G[f"{model}_5_buf{iold}"])
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_config1_l2_norm_against_reference(oracle, model):
+    """orc_l2_norm of the oracle's lagged macros (update_macros after 1000 steps) against
+    the analytic field at t = 1000 equals the golden l2_lag computed from the reference's
+    own macros (ref_config1.json)."""
+    C = json.loads((GOLDEN / "ref_config1.json").read_text())
+    n, u0, nu, steps = C["n"], C["u0"], C["nu"], C["steps"]
+    k = 2 * np.pi / n
+    p, ux, uy = oracle.tgv(n, n, k, k, u0, nu, 0.0)
+    rho = np.asfortranarray(p / (1.0 / 3.0) + 1.0)
+    par = oracle.set_properties(nu, 1.0, 0.25)
+    fa, fb = po.new_f(n, n), po.new_f(n, n)
+    oracle.set_pdf_to_equilibrium(n, n, rho, ux, uy, fa)
+    latest, prev = oracle.run(model, n, n, fa, fb, steps, par["omega"], par["trt_magic"])
+    _, u, v = oracle.update_macros(n, n, prev)
+    _, ua, va = oracle.tgv(n, n, k, k, u0, nu, float(steps))
+    assert oracle.l2_norm(u, v, ua, va) == pytest.approx(C["models"][model]["l2_lag"], rel=1e-13)
