@@ -46,8 +46,6 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-rr", action="store_true",
                     help="skip the leg of the other collisions (BGK, RR)")
-    ap.add_argument("--rr-acc-steps", type=int, default=1000,
-                    help="steps of that leg's FP32-vs-FP64 error check (0 to skip)")
     ap.add_argument("--next-steps", type=int, default=50,
                     help="timed steps of the DUGKS and FV/FD streaming legs (0 skips them)")
     return ap.parse_args()
@@ -75,18 +73,35 @@ def timed_passes(torch, lat, model, steps_per_pass, launches):
     return e0.elapsed_time(e1) / launches
 
 
-def timed_run(P, torch, lat, model, steps, warmup):
+def timed_run(P, torch, lat, model, steps, warmup, launches=None):
     """Warm up, then time exactly `steps` fused steps with HIP events on the stream the
-    kernels are launched on.  Returns (ms per step, final iold)."""
+    kernels are launched on.  Returns (ms per step, final iold).  With a list for
+    `launches`, the engine also records an event after each of its launches in the timed
+    run, and the list receives (steps advanced, ms) per launch."""
     iold, inew = 0, 1
     iold, inew = lat.run(model, warmup, iold, inew)
     torch.cuda.synchronize()
+    if launches is not None:
+        lat.set_run_timing(True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     iold, inew = lat.run(model, steps, iold, inew)
     e1.record()
     torch.cuda.synchronize()
+    if launches is not None:
+        launches.extend(lat.run_launch_times())
+        lat.set_run_timing(False)
     return e0.elapsed_time(e1) / steps, iold
+
+
+def dominant_launch(launches):
+    """Of the timed region's launches, the depth whose launches take the most time in
+    total: (depth, launches of it, their mean ms)."""
+    by = {}
+    for d, ms in launches:
+        by.setdefault(d, []).append(ms)
+    d = max(by, key=lambda k: sum(by[k]))
+    return d, len(by[d]), sum(by[d]) / len(by[d])
 
 
 def gpu_leg(args):
@@ -118,9 +133,13 @@ def gpu_leg(args):
         lat.autotune("trt", 0, reps=5)
         block, remap = lat.get_launch("trt")
         ns = lat.get_steps_per_launch("trt")
-        ms, _ = timed_run(P, torch, lat, "trt", args.steps, args.warmup)
+        launches = []
+        ms, _ = timed_run(P, torch, lat, "trt", args.steps, args.warmup, launches)
         mlups = cells / (ms * 1e-3) / 1e6
-        # the dominant kernel (a pass of ns steps) and the single-step kernel, timed alone
+        # the timed region's launches (HIP events between them on the engine's stream),
+        # the kernel that dominates it, and, timed alone, a pass of the tuned depth and the
+        # single-step kernel
+        dom, dom_n, dom_ms = dominant_launch(launches)
         pass_ms = timed_passes(torch, lat, "trt", ns, 20)
         single_ms = pass_ms if ns == 1 else timed_passes(torch, lat, "trt", 1, 20)
         bpl = BYTES_PER_LU[storage] * cells  # algorithmic bytes of one pass launch
@@ -128,6 +147,10 @@ def gpu_leg(args):
                             "steps_per_pass": ns, "pass_ms": pass_ms,
                             "pass_gbs": bpl / (pass_ms * 1e-3) / 1e9,
                             "bytes_per_lu": BYTES_PER_LU[storage] / ns,
+                            "timed_launches": [[d, round(t, 5)] for d, t in launches],
+                            "dominant_depth": dom, "dominant_launches": dom_n,
+                            "dominant_ms": dom_ms,
+                            "dominant_gbs": bpl / (dom_ms * 1e-3) / 1e9,
                             "single_step_ms": single_ms,
                             "single_step_gbs": bpl / (single_ms * 1e-3) / 1e9,
                             "launch": {"block": block, "xcd_remap": remap}}
@@ -136,33 +159,15 @@ def gpu_leg(args):
 
     # accuracy of the shifted forms vs the FP64 run (reference arithmetic) at acc-steps
     if args.acc_steps > 0:
-        macros = {}
-        for storage in ("f64", "f32s", "f16s"):
-            lat = P.Lattice(n, n, 2, storage)
-            lat.set_stream(stream)
-            lat.set_params(omega, MAGIC)
-            lat.equilibrium_init_dev(0, d_rho.data_ptr(), d_ux.data_ptr(), d_uy.data_ptr())
-            iold, _ = lat.run("trt", args.acc_steps, 0, 1)
-            m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
-            lat.macros_dev(iold, *(t.data_ptr() for t in m))
-            torch.cuda.synchronize()
-            macros[storage] = m
-            lat.close()
-        r64, u64, v64 = macros["f64"]
-        den = torch.sum(u64 * u64 + v64 * v64)
-        m64 = torch.sum(r64)
-        for storage in ("f32s", "f16s"):
-            r, u, v = macros[storage]
-            err = torch.sqrt(torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
-            results[storage]["vel_rel_l2_vs_f64"] = err
-            results[storage]["mass_rel_diff_vs_f64"] = abs((torch.sum(r) - m64) / m64).item()
-            results[storage]["acc_steps"] = args.acc_steps
-        results["f64"]["mass_drift_rel"] = abs((m64 - torch.sum(d_rho)) / torch.sum(d_rho)).item()
-        del macros
+        acc = accuracy(P, torch, stream, "trt", omega, (d_rho, d_ux, d_uy), n, args.acc_steps)
+        for storage, v in acc.items():
+            results[storage].update(v)
         torch.cuda.empty_cache()
 
     copy_gbs = P.copy_bandwidth(4 << 30, 20)
     extras = {
+        "config1": config1_leg(P, torch, stream),
+        "config2": config2_leg(P, torch, stream) if n == N else None,
         "collisions": None if args.no_rr else collisions_leg(P, torch, args, stream, omega,
                                                              (d_rho, d_ux, d_uy)),
         "schemes": schemes_leg(P, torch, args, stream, (d_rho, d_ux, d_uy))
@@ -173,18 +178,56 @@ def gpu_leg(args):
     return results, copy_gbs, dev_name, extras
 
 
+def accuracy(P, torch, stream, model, omega, fields, n, steps):
+    """FP64, FP32- and FP16-shifted runs of `steps` steps from the same start: each
+    shifted run's velocity relative L2 against the FP64 run (bit-identical to the
+    reference arithmetic), every run's relative mass drift against its own start, and the
+    shifted runs' relative mass difference to the FP64 run (FP64 reductions, plbm_reduce)."""
+    cells = n * n
+    macros, out = {}, {}
+    for storage in ("f64", "f32s", "f16s"):
+        lat = P.Lattice(n, n, 2, storage)
+        lat.set_stream(stream)
+        lat.set_params(omega, MAGIC)
+        lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
+        m0 = lat.reduce(0)[P._abi.RED_MASS]
+        iold, _ = lat.run(model, steps, 0, 1)
+        red = lat.reduce(iold)
+        m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
+        lat.macros_dev(iold, *(t.data_ptr() for t in m))
+        torch.cuda.synchronize()
+        macros[storage] = m
+        out[storage] = {"mass_drift_rel": abs(red[P._abi.RED_MASS] - m0) / m0,
+                        "nonfinite_cells": int(red[P._abi.RED_NONFINITE]),
+                        "acc_steps": steps, "_mass": red[P._abi.RED_MASS]}
+        lat.close()
+    r64, u64, v64 = macros["f64"]
+    den = torch.sum(u64 * u64 + v64 * v64)
+    for storage in ("f32s", "f16s"):
+        r, u, v = macros[storage]
+        out[storage]["vel_rel_l2_vs_f64"] = torch.sqrt(
+            torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
+        out[storage]["mass_rel_diff_vs_f64"] = abs(
+            out[storage]["_mass"] - out["f64"]["_mass"]) / out["f64"]["_mass"]
+    for v in out.values():
+        del v["_mass"]
+    del macros
+    torch.cuda.empty_cache()
+    return out
+
+
 def collisions_leg(P, torch, args, stream, omega, fields):
     """The other collision models on the same 8192^2 start: BGK (src/collision_bgk.F90)
-    and recursive-regularized (src/collision_regularized.F90), FP64 and FP32-shifted MLUPS
-    through plbm_run (autotuned), and the FP32-shifted velocity error against the FP64 run
-    after rr_acc_steps steps."""
+    and recursive-regularized (src/collision_regularized.F90), FP64, FP32- and
+    FP16-shifted MLUPS through plbm_run (autotuned), and each shifted form's velocity
+    error and mass drift against the FP64 run after acc_steps steps (5000, as the TRT
+    leg)."""
     n = args.n
     cells = n * n
     out = {}
     for model in ("bgk", "rr"):
         res = {"grid": [n, n], "steps": args.steps}
-        macros = {}
-        for storage in ("f64", "f32s"):
+        for storage in ("f64", "f32s", "f16s"):
             lat = P.Lattice(n, n, 2, storage)
             lat.set_stream(stream)
             lat.set_params(omega, MAGIC)
@@ -194,25 +237,109 @@ def collisions_leg(P, torch, args, stream, omega, fields):
             ms, _ = timed_run(P, torch, lat, model, args.steps, args.warmup)
             res[storage] = {"ms_per_step": round(ms, 5), "mlups": round(cells / ms / 1e3, 1),
                             "steps_per_pass": lat.get_steps_per_launch(model)}
-            if args.rr_acc_steps > 0:
-                lat.equilibrium_init_dev(0, *(t.data_ptr() for t in fields))
-                iold, _ = lat.run(model, args.rr_acc_steps, 0, 1)
-                m = [torch.empty(cells, dtype=torch.float64, device="cuda") for _ in range(3)]
-                lat.macros_dev(iold, *(t.data_ptr() for t in m))
-                torch.cuda.synchronize()
-                macros[storage] = m
             lat.close()
             torch.cuda.empty_cache()
-        res["f32s_speedup_vs_f64"] = round(res["f32s"]["mlups"] / res["f64"]["mlups"], 3)
-        if macros:
-            (r64, u64, v64), (r, u, v) = macros["f64"], macros["f32s"]
-            den = torch.sum(u64 * u64 + v64 * v64)
-            res["f32s"]["vel_rel_l2_vs_f64"] = torch.sqrt(
-                torch.sum((u - u64) ** 2 + (v - v64) ** 2) / den).item()
-            res["f32s"]["acc_steps"] = args.rr_acc_steps
-        del macros
+        for storage in ("f32s", "f16s"):
+            res[f"{storage}_speedup_vs_f64"] = round(
+                res[storage]["mlups"] / res["f64"]["mlups"], 3)
+        if args.acc_steps > 0:
+            for storage, v in accuracy(P, torch, stream, model, omega, fields, n,
+                                       args.acc_steps).items():
+                res[storage].update(v)
         out[model] = res
     return out
+
+
+def config1_leg(P, torch, stream):
+    """The correctness configuration: 64^2 Taylor-Green (u0 = 0.05, nu = 0.01, k = 2 pi/64,
+    rho = 1 + 3p, equilibrium start), 1000 steps through plbm_run, FP64, for BGK (the
+    configuration's model) and TRT / RR.  Reports whether both slots are bit-identical to
+    the reference Fortran's buffers (sha256 fingerprints in tests/golden/ref_config1.json,
+    written by the flang-built reference), the app's relative L2 norm of the lagged state
+    against the analytic field at t = 1000 (app/main_taylor_green.f90:174-212) beside the
+    reference's value, the kinetic-energy decay against the analytic exp(-2 nu k^2 t) per
+    velocity component, and the mass drift."""
+    import hashlib
+    import numpy as np
+    G = json.loads((ROOT / "tests" / "golden" / "ref_config1.json").read_text())
+    n, u0, nu, steps = G["n"], G["u0"], G["nu"], G["steps"]
+    k = 2 * np.pi / n
+    p, ux, uy = P.TaylorGreen(n, n, k, k, u0, nu).eval(0.0)
+    rho = np.asfortranarray(p / P.csqr + 1.0)
+    _, uxa, uya = P.TaylorGreen(n, n, k, k, u0, nu).eval(float(steps))
+    omega = 1.0 / (3.0 * nu + 0.5)
+    sha = lambda a: hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+    out = {"grid": [n, n], "steps": steps, "u0": u0, "nu": nu}
+    for model in ("bgk", "trt", "rr"):
+        lat = P.Lattice(n, n, 2, "f64")
+        lat.set_stream(stream)
+        lat.set_params(omega, MAGIC)
+        lat.equilibrium_init(0, rho, ux, uy)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        iold, inew = lat.run(model, steps, 0, 1)
+        e1.record()
+        torch.cuda.synchronize()
+        latest, prev = lat.download_f(iold)[:n], lat.download_f(inew)[:n]
+        r, u, v = lat.macros(inew)  # update_macros reads the lagged slot
+        lat.close()
+        M = G["models"][model]
+        l2 = float(np.linalg.norm(np.hypot(u - uxa, v - uya)) / np.linalg.norm(np.hypot(uxa, uya)))
+        ke = float(np.sum(u ** 2 + v ** 2) / np.sum(ux ** 2 + uy ** 2))
+        out[model] = {
+            "bit_identical_to_reference": sha(latest) == M["sha256_latest"]
+            and sha(prev) == M["sha256_prev"],
+            "l2_lag_vs_analytic": l2, "reference_l2_lag": M["l2_lag"],
+            # the lagged state is at step steps - 1: u decays as exp(-2 nu k^2 t)
+            "ke_ratio": ke, "analytic_ke_ratio": float(np.exp(-4 * nu * k * k * (steps - 1))),
+            "mass_drift_rel": abs(float(np.sum(r)) - float(np.sum(rho))) / float(np.sum(rho)),
+            "ms_per_step": round(e0.elapsed_time(e1) / steps, 5)}
+    return out
+
+
+def config2_leg(P, torch, stream, steps=2000):
+    """The FP64 roofline configuration: 4096^2 TRT (Lambda = 1/4), Taylor-Green u0 = 0.04,
+    nu = 0.005 plus U(+-1e-3) noise of seed 1, equilibrium start, `steps` steps through
+    plbm_run (autotuned).  MLUPS, the pass kernel's and the single-step kernel's achieved
+    algorithmic bandwidth (144 B per cell per launch) against the 8 TB/s peak."""
+    import numpy as np
+    n, u0, nu = 4096, 0.04, 0.005
+    k = 2 * np.pi / n
+    p, ux, uy = P.TaylorGreen(n, n, k, k, u0, nu).eval(0.0)
+    P.add_noise(ux, uy, 1, NOISE)
+    rho = np.asfortranarray(p / P.csqr + 1.0)
+    omega = 1.0 / (3.0 * nu + 0.5)
+    lat = P.Lattice(n, n, 2, "f64")
+    lat.set_stream(stream)
+    lat.set_params(omega, MAGIC)
+    lat.equilibrium_init(0, rho, ux, uy)
+    torch.cuda.synchronize()
+    lat.autotune("trt", 0, reps=5)
+    ns = lat.get_steps_per_launch("trt")
+    lat.equilibrium_init(0, rho, ux, uy)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    iold, _ = lat.run("trt", steps, 0, 1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    red = lat.reduce(iold)
+    pass_ms = timed_passes(torch, lat, "trt", ns, 20)
+    single_ms = timed_passes(torch, lat, "trt", 1, 20)
+    bpl = BYTES_PER_LU["f64"] * n * n
+    lat.close()
+    torch.cuda.empty_cache()
+    return {"grid": [n, n], "steps": steps, "ms_per_step": round(ms, 5),
+            "mlups": round(n * n / ms / 1e3, 1), "steps_per_pass": ns,
+            "pass_kernel": f"k_multi<TRT, F64, {ns}>", "pass_ms": round(pass_ms, 5),
+            "pass_gbs": round(bpl / pass_ms / 1e6, 1),
+            "pass_frac_of_peak": round(bpl / pass_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "single_step_ms": round(single_ms, 5),
+            "single_step_gbs": round(bpl / single_ms / 1e6, 1),
+            "single_step_frac_of_peak": round(bpl / single_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "nonfinite_cells": int(red[P._abi.RED_NONFINITE])}
 
 
 def schemes_leg(P, torch, args, stream, fields):
@@ -381,15 +508,46 @@ def adjoint_leg(P, torch, args):
     return out
 
 
+def host_cpu():
+    """CPU model, the host's physical cores (distinct (package, core) pairs) and logical
+    CPUs, and the CPUs this process may run on, from /proc/cpuinfo and the affinity mask."""
+    model, pairs, logical = "unknown", set(), 0
+    try:
+        phys = core = None
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "processor":
+                logical += 1
+            elif key == "model name" and model == "unknown":
+                model = val
+            elif key == "physical id":
+                phys = val
+            elif key == "core id":
+                core = val
+                pairs.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "host_physical_cores": len(pairs) or None,
+            "host_logical_cpus": logical or os.cpu_count(), "cpus_allowed": allowed}
+
+
 def cpu_leg(args):
     """The reference Fortran (oracle/_ref, flang -O2 -fopenmp) on the host cores, over a
-    bounded sample of the same workload: the 8192^2 TRT grid for --cpu-steps steps."""
+    bounded sample of the same workload: the 8192^2 TRT grid for --cpu-steps steps, with
+    OpenMP over this process's share of the host (OMP_NUM_THREADS; 16 per GPU on a box
+    shared per GPU), CPU model and core counts stated."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
     import __graft_entry__ as g
     P = g.load_package()
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, cpu["cpus_allowed"])
     n = args.n
     rho, ux, uy = initial_fields(P, n)
     if po.have_ref():
@@ -419,7 +577,8 @@ def cpu_leg(args):
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": cores, "kind": kind,
             "sample": f"D2Q9 TRT {n}x{n} FP64, {args.cpu_steps} steps after 1 warm-up step "
                       f"({'flang-built reference Fortran' if kind == 'reference' else 'C oracle'}, "
-                      f"OpenMP {cores} threads), {dt:.1f} s"}
+                      f"OpenMP {cores} threads), {dt:.1f} s",
+            "cpu": cpu}
 
 
 def traffic_from_profiles(n, ns):
@@ -449,8 +608,9 @@ def main():
     f64 = results["f64"]
     n = args.n
     ns = f64["steps_per_pass"]
+    dom = f64["dominant_depth"]
     bytes_per_launch = BYTES_PER_LU["f64"] * n * n  # one read + one write per population
-    achieved = bytes_per_launch / (f64["pass_ms"] * 1e-3) / 1e9
+    achieved = bytes_per_launch / (f64["dominant_ms"] * 1e-3) / 1e9
     line = {
         "metric": "MLUPS & % HBM roofline, D2Q9 TRT 8192^2: FP64 vs FP32/FP16-shifted at stated err",
         "value": round(f64["mlups"], 1),
@@ -465,15 +625,23 @@ def main():
         "dtype": "f64",
         "data": "synthetic (Taylor-Green u0=0.05 nu=0.01 + U(+-1e-3) noise, seed 7)",
         "config": {"workload": f"D2Q9 TRT Lambda=1/4 periodic {n}x{n}, FP64 populations, "
-                               f"fused pull-stream+collide, {ns} steps per HBM pass",
+                               f"fused pull-stream+collide, passes of up to {ns} steps per "
+                               f"HBM sweep",
                    "grid": [n, n], "model": "trt", "storage": "f64",
                    "omega": 1.0 / (3.0 * NU + 0.5), "device": dev_name},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(n, ns),
-                     "kernel": f"k_multi<TRT, F64, {ns}>" if ns > 1 else "k_fused<TRT, F64>",
-                     "launch_ms": round(f64["pass_ms"], 5), "lu_per_launch": ns * n * n,
-                     "bytes_per_launch": bytes_per_launch},
+                     "traffic": traffic_from_profiles(n, dom),
+                     "kernel": f"k_multi<TRT, F64, {dom}>" if dom > 1 else "k_fused<TRT, F64>",
+                     "launch_ms": round(f64["dominant_ms"], 5),
+                     "launches_in_timed_region": f64["dominant_launches"],
+                     "lu_per_launch": dom * n * n, "bytes_per_launch": bytes_per_launch,
+                     "timed_launches": f64["timed_launches"]},
+        "tuned_depth_pass": {
+            "kernel": f"k_multi<TRT, F64, {ns}>" if ns > 1 else "k_fused<TRT, F64>",
+            "launch_ms": round(f64["pass_ms"], 5), "achieved": round(f64["pass_gbs"], 1),
+            "frac": round(f64["pass_gbs"] / HBM_PEAK_GBS, 4), "ms_per_step": round(
+                f64["pass_ms"] / ns, 5)},
         "single_step_kernel": {
             "kernel": "k_fused<TRT, F64>", "launch_ms": round(f64["single_step_ms"], 5),
             "achieved": round(f64["single_step_gbs"], 1),
@@ -500,6 +668,8 @@ def main():
             for k, v in results.items()},
         "f32s_speedup_vs_f64": round(results["f32s"]["mlups"] / f64["mlups"], 3),
         "f16s_speedup_vs_f64": round(results["f16s"]["mlups"] / f64["mlups"], 3),
+        "config1_taylor_green_64": extras["config1"],
+        "config2_roofline_4096": extras["config2"],
         "other_collisions": extras["collisions"],
         "batched_sweep": extras["batched_sweep"],
         "adjoint": extras["adjoint"],

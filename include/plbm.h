@@ -213,7 +213,8 @@ const char *plbm_last_error(void);
 /* taylor_green_t%eval (src/benchmarks/taylor_green.f90:31-84) on an (ny, nx) grid. */
 int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, double nu,
                             double t, double *p, double *ux, double *uy);
-/* shear_layer_t%eval (src/benchmarks/shear_layer.f90:123-150). */
+/* shear_layer_t's grid evaluation eval_shear_layer_grid
+ * (src/benchmarks/shear_layer.f90:25-52). */
 int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
                            double *rho, double *ux, double *uy);
 /* vortex_case_t%eval (src/benchmarks/barotropic_vortex_case.F90:34-83): convected
@@ -267,6 +268,16 @@ int plbm_get_pass_segments(const plbm_lattice *h, int *segments);
  * buffer is freed and runs end on a single-step launch instead (one more sweep over
  * HBM per run, two buffers of memory).  Results are identical either way. */
 int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable);
+/* Launch timing of plbm_run (engine extension, no counterpart in the reference): with
+ * enable = 1 every later plbm_run records HIP events on the lattice's stream before its
+ * first launch and after each launch.  plbm_run_launch_times then waits for the last run
+ * and returns, per launch in order, the steps it advanced (depths[k]; 1 = the single-step
+ * kernel) and its duration in ms (ms[k], between consecutive events); *nlaunches = the
+ * number of launches (up to max_launches entries are written; either array may be
+ * NULL).  PLBM_ERR_STATE when timing is off. */
+int plbm_set_run_timing(plbm_lattice *h, int enable);
+int plbm_run_launch_times(plbm_lattice *h, int max_launches, int *depths, float *ms,
+                          int *nlaunches);
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
  * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
  * of 2..8 steps against it per lattice update; keep the fastest for this lattice and

@@ -4,8 +4,8 @@
  * below is parenthesised exactly as the Fortran source evaluates it (Fortran evaluates
  * a chain of equal-precedence operators left to right unless parentheses say otherwise,
  * and unary minus binds looser than '*').  That is what makes the FP64 results
- * bit-identical to the flang-built reference; tests/test_oracle_vs_reference.py checks
- * it bit for bit, and tests/test_oracle_invariants.py pins each function independently
+ * bit-identical to the flang-built reference; tests/test_oracle_golden.py checks it bit
+ * for bit against the fixtures that reference wrote (tests/golden/), and tests/test_oracle_invariants.py pins each function independently
  * (moment identities, permutation property, conservation, closed forms).
  */
 #include "oracle.h"
@@ -685,7 +685,7 @@ void orc_tgv_eval(int nx, int ny, double kx, double ky, double umax, double nu, 
     }
 }
 
-/* src/benchmarks/shear_layer.f90:123-150 (eval_shear_layer_grid). */
+/* src/benchmarks/shear_layer.f90:25-52 (eval_shear_layer_grid). */
 void orc_shear_eval(int nx, int ny, double length, double u0, double k, double delta,
                     double *rho, double *ux, double *uy) {
     const double pi = 4.0 * atan(1.0);

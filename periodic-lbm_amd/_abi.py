@@ -118,6 +118,8 @@ SIGNATURES = {
     "plbm_set_pass_segments": (_i, [_vp, _i]),
     "plbm_get_pass_segments": (_i, [_vp, _ip]),
     "plbm_set_pass_lag_buffer": (_i, [_vp, _i]),
+    "plbm_set_run_timing": (_i, [_vp, _i]),
+    "plbm_run_launch_times": (_i, [_vp, _i, _ip, ctypes.POINTER(ctypes.c_float), _ip]),
     "plbm_dugks_collide": (_i, [_vp, _i, _i, _d, _d]),
     "plbm_dugks_stream": (_i, [_vp, _i, _i, _d]),
     "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),

@@ -32,6 +32,7 @@
 #include "common.hpp"
 #include "d2q9.hpp"
 #include "multistep.hpp"
+#include "reduce.hpp"
 #include "stepper.hpp"
 
 namespace plbm {
@@ -160,20 +161,22 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
-// Per-case FP64 reductions of the buffer holding the latest state (one block per case).
+// Per-case FP64 reductions of the buffer holding the latest state (one workgroup per
+// case, reduce.hpp).
 template <int ST>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRedThreads)
     k_batch_reduce(const BatchCase *__restrict__ cases, const void *arena, int parity,
                    float inv_scale, double *out) {
     using T = typename Traits<ST>::T;
-    __shared__ double sh[256][PLBM_NRED];
     const BatchCase c = cases[blockIdx.x];
     const T *s = static_cast<const T *>(arena) + c.off + (parity ? c.elems : 0);
     double acc[PLBM_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < c.nx * c.ny; i += blockDim.x) {
-        const int x = i / c.ny, y = i % c.ny;
+    const int n = c.nx * c.ny, sx = kRedThreads / c.ny, sy = kRedThreads % c.ny;
+    CellWalk w;
+    w.start(threadIdx.x, c.ny);
+    for (int i = threadIdx.x; i < n; i += kRedThreads) {
         double f[9];
-        const size_t base = pop_index(0, x, y, c.ldd);
+        const size_t base = pop_index(0, w.x, w.y, c.ldd);
 #pragma unroll
         for (int q = 0; q < 9; ++q) f[q] = double(load_pop<ST>(s + base + (size_t)q * c.ldd, inv_scale));
         double rho, ux, uy;
@@ -186,28 +189,11 @@ __global__ void __launch_bounds__(256)
             ux = jx / rho;
             uy = jy / rho;
         }
-        if (!(__builtin_isfinite(rho) && __builtin_isfinite(ux) && __builtin_isfinite(uy))) {
-            acc[PLBM_RED_NONFINITE] += 1.0;
-            continue;
-        }
-        const double u2 = ux * ux + uy * uy;
-        acc[PLBM_RED_MASS] += rho;
-        acc[PLBM_RED_KE] += 0.5 * rho * u2;
-        acc[PLBM_RED_U2] += u2;
-        acc[PLBM_RED_UMAX] = fmax(acc[PLBM_RED_UMAX], sqrt(u2));
+        red_cell(acc, rho, ux, uy, nullptr, nullptr, 0);
+        w.next(sx, sy, c.ny);
     }
-    for (int k = 0; k < PLBM_NRED; ++k) sh[threadIdx.x][k] = acc[k];
-    __syncthreads();
-    for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
-        if ((int)threadIdx.x < stride)
-            for (int k = 0; k < PLBM_NRED; ++k) {
-                const double o = sh[threadIdx.x + stride][k];
-                sh[threadIdx.x][k] = k == PLBM_RED_UMAX ? fmax(sh[threadIdx.x][k], o)
-                                                         : sh[threadIdx.x][k] + o;
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x < PLBM_NRED) out[(size_t)blockIdx.x * PLBM_NRED + threadIdx.x] = sh[0][threadIdx.x];
+    const double v = red_block(acc);
+    if (threadIdx.x < PLBM_NRED) out[(size_t)blockIdx.x * PLBM_NRED + threadIdx.x] = v;
 }
 
 // Host-layout <-> arena conversion of one q-plane of one case (as in the single lattice).
@@ -683,7 +669,7 @@ int plbm_batch_macros(plbm_batch *b, int c, double *rho, double *ux, double *uy)
 int plbm_batch_reduce(plbm_batch *b, double *out) {
     PLBM_REQUIRE(b && out, PLBM_ERR_ARG, "null argument");
     PLBM_TRY(dispatch_storage(b->storage, [&](auto S) {
-        hipLaunchKernelGGL(k_batch_reduce<decltype(S)::value>, dim3(b->ncases), dim3(256), 0,
+        hipLaunchKernelGGL(k_batch_reduce<decltype(S)::value>, dim3(b->ncases), dim3(kRedThreads), 0,
                            b->stream, b->d_cases, b->arena, b->parity, float(1.0 / b->scale),
                            b->d_red);
         PLBM_HIP(hipGetLastError());
@@ -800,14 +786,21 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
     PLBM_REQUIRE(b->params_set, PLBM_ERR_STATE, "call plbm_batch_set_params first");
     PLBM_HIP(hipStreamSynchronize(b->stream));
     void *scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = on_scope_exit([&] {
+        (void)hipStreamSynchronize(b->side);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (scratch) (void)hipFree(scratch);
+    });
     const size_t bytes = b->arena_elems * storage_bytes(b->storage);
     if (hipMalloc(&scratch, bytes) != hipSuccess) {
         (void)hipGetLastError();
+        scratch = nullptr;
         set_error("batch autotune: no memory for a scratch arena");
         return PLBM_ERR_NOMEM;
     }
     PLBM_HIP(hipMemcpy(scratch, b->arena, bytes, hipMemcpyDeviceToDevice));
-    hipEvent_t e0, e1;
     PLBM_HIP(hipEventCreate(&e0));
     PLBM_HIP(hipEventCreate(&e1));
     std::vector<BatchCfg> cands;
@@ -870,10 +863,6 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
             best = c;
         }
     }
-    (void)hipStreamSynchronize(b->side);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(scratch);
     if (status != PLBM_OK) return status;
     PLBM_TRY(set_cfg(b, best));
     if (best_ms) *best_ms = tbest;

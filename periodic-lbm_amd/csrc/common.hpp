@@ -41,6 +41,18 @@ const char *last_error();
         if (s_ != PLBM_OK) return s_;                                                     \
     } while (0)
 
+// Runs f when the guard leaves scope: releases device buffers and events on every return
+// path of a function (PLBM_HIP / PLBM_TRY return early).
+template <class F>
+struct ScopeExit {
+    F f;
+    ~ScopeExit() { f(); }
+};
+template <class F>
+ScopeExit<F> on_scope_exit(F f) {
+    return ScopeExit<F>{f};
+}
+
 // Storage traits: T is what sits in HBM, R is the arithmetic type of the collision.
 template <int ST> struct Traits;
 template <> struct Traits<PLBM_F64> {
@@ -182,6 +194,11 @@ struct plbm_lattice {
     void *aux = nullptr;
     bool aux_failed = false;             // its allocation failed once; runs end on a single step
     bool aux_enabled = true;             // plbm_set_pass_lag_buffer
+    // per-launch timing of plbm_run (plbm_set_run_timing): events around each launch of
+    // the last run and the steps each launch advanced
+    bool run_timing = false;
+    std::vector<hipEvent_t> run_events;
+    std::vector<int> run_depths;
     // scratch (allocated on first use)
     double *d_tmp = nullptr;       // staging for host transfers
     size_t tmp_bytes = 0;

@@ -39,7 +39,7 @@ int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, d
     return PLBM_OK;
 }
 
-// eval_shear_layer_grid, src/benchmarks/shear_layer.f90:123-150.
+// eval_shear_layer_grid, src/benchmarks/shear_layer.f90:25-52.
 int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
                            double *rho, double *ux, double *uy) {
     if (nx < 1 || ny < 1 || !rho || !ux || !uy || !(length > 0)) return PLBM_ERR_ARG;

@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "d2q9.hpp"
+#include "reduce.hpp"
 #include "stepper.hpp"
 
 namespace plbm {
@@ -136,60 +137,39 @@ __global__ void __launch_bounds__(256) k_macros(const void *src, int nx, int ny,
     uy[o] = v;
 }
 
-// Deterministic two-pass FP64 reduction: fixed cell->thread map (grid-stride over the
-// column-major cell index), fixed LDS tree per block, one block for the final pass.
+// Deterministic two-pass FP64 reduction (reduce.hpp): workgroup b walks the cells
+// b*256 + t, advancing by the grid's thread count, and writes its partial to slot b; one
+// workgroup combines the partials.
 template <int ST>
-__global__ void __launch_bounds__(256) k_reduce(const void *src, int nx, int ny, int ldd,
-                                                float inv_scale, const double *uxa,
-                                                const double *uya, double *partials) {
-    __shared__ double sh[256][PLBM_NRED];
+__global__ void __launch_bounds__(kRedThreads) k_reduce(const void *src, int nx, int ny, int ldd,
+                                                        float inv_scale, const double *uxa,
+                                                        const double *uya, int sx, int sy,
+                                                        double *partials) {
     const auto *s = static_cast<const typename Traits<ST>::T *>(src);
     double acc[PLBM_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const size_t n = (size_t)nx * ny;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i / ny), y = (int)(i % ny);
+    const long long n = (long long)nx * ny, stride = (long long)gridDim.x * kRedThreads;
+    const long long i0 = (long long)blockIdx.x * kRedThreads + threadIdx.x;
+    CellWalk w;
+    w.start(i0, ny);
+    for (long long i = i0; i < n; i += stride) {
         double rho, ux, uy;
-        cell_macros<ST>(s, ldd, x, y, inv_scale, rho, ux, uy);
-        const double u2 = ux * ux + uy * uy;
-        if (!(__builtin_isfinite(rho) && __builtin_isfinite(ux) && __builtin_isfinite(uy))) {
-            acc[PLBM_RED_NONFINITE] += 1.0;
-            continue;
-        }
-        acc[PLBM_RED_MASS] += rho;
-        acc[PLBM_RED_KE] += 0.5 * rho * u2;
-        acc[PLBM_RED_U2] += u2;
-        acc[PLBM_RED_UMAX] = fmax(acc[PLBM_RED_UMAX], sqrt(u2));
-        if (uxa) {
-            const double ex = ux - uxa[i], ey = uy - uya[i];
-            acc[PLBM_RED_ERR2] += ex * ex + ey * ey;
-            acc[PLBM_RED_REF2] += uxa[i] * uxa[i] + uya[i] * uya[i];
-        }
+        cell_macros<ST>(s, ldd, w.x, w.y, inv_scale, rho, ux, uy);
+        red_cell(acc, rho, ux, uy, uxa, uya, (size_t)i);
+        w.next(sx, sy, ny);
     }
-    for (int k = 0; k < PLBM_NRED; ++k) sh[threadIdx.x][k] = acc[k];
-    __syncthreads();
-    for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
-        if ((int)threadIdx.x < stride) {
-            for (int k = 0; k < PLBM_NRED; ++k) {
-                const double o = sh[threadIdx.x + stride][k];
-                sh[threadIdx.x][k] = k == PLBM_RED_UMAX ? fmax(sh[threadIdx.x][k], o)
-                                                         : sh[threadIdx.x][k] + o;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < PLBM_NRED) partials[blockIdx.x * PLBM_NRED + threadIdx.x] = sh[0][threadIdx.x];
+    const double v = red_block(acc);
+    if (threadIdx.x < PLBM_NRED) partials[(size_t)blockIdx.x * PLBM_NRED + threadIdx.x] = v;
 }
 
-__global__ void __launch_bounds__(PLBM_NRED) k_reduce_final(const double *partials, int nblocks,
-                                                            double *out) {
-    const int k = threadIdx.x;
-    double a = 0.0;
-    for (int b = 0; b < nblocks; ++b) {
-        const double v = partials[b * PLBM_NRED + k];
-        a = k == PLBM_RED_UMAX ? fmax(a, v) : a + v;
-    }
-    out[k] = a;
+__global__ void __launch_bounds__(kRedThreads) k_reduce_parts(const double *partials, int nparts,
+                                                              double *out) {
+    double acc[PLBM_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += kRedThreads)
+#pragma unroll
+        for (int k = 0; k < PLBM_NRED; ++k)
+            acc[k] = red_combine(k, acc[k], partials[(size_t)b * PLBM_NRED + k]);
+    const double v = red_block(acc);
+    if (threadIdx.x < PLBM_NRED) out[threadIdx.x] = v;
 }
 
 // set_pdf_to_equilibrium: buffer = feq(rho, u).  Shifted forms are evaluated in FP64 and
@@ -302,6 +282,33 @@ static int ensure_tmp(plbm_lattice *h, size_t bytes) {
     return PLBM_OK;
 }
 
+// Workgroups of the reduction of an nx x ny grid: one per 256 cells, at most
+// kReduceBlocks (the partial slots); each thread then walks every 256*nb-th cell.
+static int reduce_blocks(const plbm_lattice *h) {
+    const size_t n = (size_t)h->nx * h->ny;
+    return (int)std::min<size_t>(kReduceBlocks, (n + kRedThreads - 1) / kRedThreads);
+}
+
+// The two reduction kernels on the lattice's stream: partials of reduce_blocks(h)
+// workgroups into parts, their combination into out_dev (PLBM_NRED doubles, device).
+static int launch_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                         double *parts, double *out_dev) {
+    const int nb = reduce_blocks(h);
+    const long long stride = (long long)nb * kRedThreads;
+    const int sx = (int)(stride / h->ny), sy = (int)(stride % h->ny);
+    PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
+        hipLaunchKernelGGL(k_reduce<decltype(S)::value>, dim3(nb), dim3(kRedThreads), 0,
+                           h->stream, h->f[buf], h->nx, h->ny, h->ldd, float(1.0 / h->scale),
+                           uxa, uya, sx, sy, parts);
+        PLBM_HIP(hipGetLastError());
+        return PLBM_OK;
+    }));
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kRedThreads), 0, h->stream, parts, nb,
+                       out_dev);
+    PLBM_HIP(hipGetLastError());
+    return PLBM_OK;
+}
+
 static dim3 cell_grid(const plbm_lattice *h) {
     return dim3((unsigned)((h->ny + 255) / 256), (unsigned)h->nx);
 }
@@ -370,13 +377,19 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
         }
     }
     void *scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = on_scope_exit([&] {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (scratch) (void)hipFree(scratch);
+    });
     const size_t bytes = h->elems * storage_bytes(h->storage);
     if (hipMalloc(&scratch, bytes) != hipSuccess) {
         (void)hipGetLastError();
+        scratch = nullptr;
         set_error("autotune: no memory for a scratch buffer");
         return PLBM_ERR_NOMEM;
     }
-    hipEvent_t e0, e1;
     PLBM_HIP(hipEventCreate(&e0));
     PLBM_HIP(hipEventCreate(&e1));
     LaunchCfg best;
@@ -436,9 +449,6 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
             best.ns = ns;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(scratch);
     if (status != PLBM_OK) return status;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -520,6 +530,7 @@ int plbm_destroy(plbm_lattice *h) {
     for (int b = 0; b < kMaxBuffers; ++b)
         if (h->f[b]) (void)hipFree(h->f[b]);
     if (h->aux) (void)hipFree(h->aux);
+    for (hipEvent_t e : h->run_events) (void)hipEventDestroy(e);
     if (h->d_tmp) (void)hipFree(h->d_tmp);
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->d_red) (void)hipFree(h->d_red);
@@ -741,18 +752,7 @@ int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *u
                  "uxa and uya must both be given or both be NULL");
     if (!h->d_partials) PLBM_HIP(hipMalloc(&h->d_partials, kReduceBlocks * PLBM_NRED * 8));
     if (!h->d_red) PLBM_HIP(hipMalloc(&h->d_red, PLBM_NRED * 8));
-    const size_t n = (size_t)h->nx * h->ny;
-    const int nb = (int)std::min<size_t>(kReduceBlocks, (n + 255) / 256);
-    PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
-        hipLaunchKernelGGL(k_reduce<decltype(S)::value>, dim3(nb), dim3(256), 0, h->stream,
-                           h->f[buf], h->nx, h->ny, h->ldd, float(1.0 / h->scale), uxa,
-                           uya, h->d_partials);
-        PLBM_HIP(hipGetLastError());
-        return PLBM_OK;
-    }));
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(PLBM_NRED), 0, h->stream, h->d_partials, nb,
-                       h->d_red);
-    PLBM_HIP(hipGetLastError());
+    PLBM_TRY(launch_reduce(h, buf, uxa, uya, h->d_partials, h->d_red));
     PLBM_HIP(hipMemcpyAsync(out, h->d_red, PLBM_NRED * 8, hipMemcpyDeviceToHost, h->stream));
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
@@ -828,20 +828,26 @@ int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms
 int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
     PLBM_REQUIRE(ms && iters > 0 && bytes >= 16, PLBM_ERR_ARG, "bad argument");
     const size_t n = bytes / 16;
+    PLBM_REQUIRE(n / 256 < (size_t)1 << 31, PLBM_ERR_ARG, "copy probe size too large");
     v4f *a = nullptr, *b = nullptr;
-    PLBM_HIP(hipMalloc(&a, n * 16));
-    if (hipMalloc(&b, n * 16) != hipSuccess) {
-        (void)hipFree(a);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = on_scope_exit([&] {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    });
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipGetLastError();
         set_error("hipMalloc failed in copy probe");
         return PLBM_ERR_NOMEM;
     }
-    hipEvent_t e0, e1;
     PLBM_HIP(hipEventCreate(&e0));
     PLBM_HIP(hipEventCreate(&e1));
     PLBM_HIP(hipMemset(a, 0, n * 16));
-    PLBM_REQUIRE(n / 256 < (size_t)1 << 31, PLBM_ERR_ARG, "copy probe size too large");
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, a, b, n);  // warm-up
+    PLBM_HIP(hipGetLastError());
     PLBM_HIP(hipEventRecord(e0, nullptr));
     for (int i = 0; i < iters; ++i)
         hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, a, b, n);
@@ -850,10 +856,6 @@ int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
     float t = 0;
     PLBM_HIP(hipEventElapsedTime(&t, e0, e1));
     *ms = double(t) / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
     return PLBM_OK;
 }
 

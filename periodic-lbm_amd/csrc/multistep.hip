@@ -183,10 +183,28 @@ void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void 
 // which becomes the other slot's storage, as update_macros expects of it (the reference's
 // inew after perform_lbm_step).  Without a third buffer the last step is a single-step
 // launch instead.
+// With run timing on, an event is recorded on the lattice's stream before the first
+// launch of a run and after each launch (plbm_run_launch_times reads them).
+static int timing_mark(plbm_lattice *h, int depth) {
+    if (!h->run_timing) return PLBM_OK;
+    const size_t k = h->run_depths.size() + (depth > 0 ? 1 : 0);
+    while (h->run_events.size() <= k) {
+        hipEvent_t e;
+        PLBM_HIP(hipEventCreate(&e));
+        h->run_events.push_back(e);
+    }
+    if (depth > 0) h->run_depths.push_back(depth);
+    PLBM_HIP(hipEventRecord(h->run_events[k], h->stream));
+    return PLBM_OK;
+}
+
 int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew) {
+    h->run_depths.clear();
+    PLBM_TRY(timing_mark(h, 0));
     if (ns <= 1 || nsteps < 2) {
         for (int s = 0; s < nsteps; ++s) {
             PLBM_TRY(launch_step(h, model, h->f[*iold], h->f[*inew]));
+            PLBM_TRY(timing_mark(h, 1));
             std::swap(*iold, *inew);
         }
         return PLBM_OK;
@@ -194,23 +212,26 @@ int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *in
     const int a0 = *iold, b0 = *inew;
     void *latest = h->f[a0], *other = h->f[b0];
     void *aux = pass_aux(h);
+    auto launch = [&](int k, void *lag) {
+        PLBM_TRY(k == 1 ? launch_step(h, model, latest, other)
+                        : launch_multi(h, model, k, latest, other, lag));
+        return timing_mark(h, k);
+    };
     if (aux == nullptr) {
         for (int k : pass_plan(nsteps - 1, ns)) {
-            if (k == 1) PLBM_TRY(launch_step(h, model, latest, other));
-            else PLBM_TRY(launch_multi(h, model, k, latest, other));
+            PLBM_TRY(launch(k, nullptr));
             std::swap(latest, other);
         }
-        PLBM_TRY(launch_step(h, model, latest, other));
+        PLBM_TRY(launch(1, nullptr));
         finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
         return PLBM_OK;
     }
     const std::vector<int> plan = pass_plan(nsteps, ns);
     for (size_t p = 0; p + 1 < plan.size(); ++p) {
-        if (plan[p] == 1) PLBM_TRY(launch_step(h, model, latest, other));
-        else PLBM_TRY(launch_multi(h, model, plan[p], latest, other));
+        PLBM_TRY(launch(plan[p], nullptr));
         std::swap(latest, other);
     }
-    PLBM_TRY(launch_multi(h, model, plan.back(), latest, other, aux));
+    PLBM_TRY(launch(plan.back(), aux));
     finish_run(h, a0, b0, nsteps, other, aux, latest, iold, inew);
     return PLBM_OK;
 }
@@ -266,6 +287,30 @@ extern "C" int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable) {
         h->aux = nullptr;
     }
     if (h->aux_enabled) h->aux_failed = false;
+    return PLBM_OK;
+}
+
+extern "C" int plbm_set_run_timing(plbm_lattice *h, int enable) {
+    PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
+    PLBM_REQUIRE(enable == 0 || enable == 1, PLBM_ERR_ARG, "enable must be 0 or 1");
+    h->run_timing = enable != 0;
+    h->run_depths.clear();
+    return PLBM_OK;
+}
+
+extern "C" int plbm_run_launch_times(plbm_lattice *h, int max_launches, int *depths, float *ms,
+                                     int *nlaunches) {
+    PLBM_REQUIRE(h && nlaunches && max_launches >= 0, PLBM_ERR_ARG, "bad argument");
+    PLBM_REQUIRE(h->run_timing, PLBM_ERR_STATE, "run timing is off (plbm_set_run_timing)");
+    const int n = (int)h->run_depths.size();
+    *nlaunches = n;
+    if (n == 0) return PLBM_OK;
+    PLBM_HIP(hipEventSynchronize(h->run_events[(size_t)n]));
+    for (int k = 0; k < n && k < max_launches; ++k) {
+        if (depths) depths[k] = h->run_depths[(size_t)k];
+        if (ms) PLBM_HIP(hipEventElapsedTime(&ms[k], h->run_events[(size_t)k],
+                                             h->run_events[(size_t)k + 1]));
+    }
     return PLBM_OK;
 }
 

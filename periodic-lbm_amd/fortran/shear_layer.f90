@@ -36,7 +36,7 @@ module shear_layer_case
 
 contains
 
-   !> eval_shear_layer_grid (src/benchmarks/shear_layer.f90:123-150)
+   !> eval_shear_layer_grid (src/benchmarks/shear_layer.f90:25-52)
    subroutine eval_shear_layer_grid(case,nx,ny,rho,ux,uy)
       class(shear_layer_t), intent(in) :: case
       integer, intent(in) :: nx, ny
@@ -48,7 +48,7 @@ contains
       rho = case%rho0
    end subroutine
 
-   !> eval_shear_layer_unstructured (src/benchmarks/shear_layer.f90:152-179): points
+   !> eval_shear_layer_unstructured (src/benchmarks/shear_layer.f90:54-81): points
    !> xy(2,n) in absolute coordinates.
    subroutine eval_shear_layer_unstructured(case,n,xy,rho,ux,uy)
       class(shear_layer_t), intent(in) :: case

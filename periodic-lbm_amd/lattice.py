@@ -118,6 +118,19 @@ class Lattice:
         check(lib().plbm_get_pass_segments(self.h, ctypes.byref(v)))
         return v.value
 
+    def set_run_timing(self, enable: bool):
+        """Record HIP events around every launch of later run() calls."""
+        check(lib().plbm_set_run_timing(self.h, int(bool(enable))))
+
+    def run_launch_times(self) -> list[tuple[int, float]]:
+        """(steps advanced, ms) of each launch of the last run() (waits for it)."""
+        n = ctypes.c_int()
+        check(lib().plbm_run_launch_times(self.h, 0, None, None, ctypes.byref(n)))
+        d = (ctypes.c_int * max(1, n.value))()
+        ms = (ctypes.c_float * max(1, n.value))()
+        check(lib().plbm_run_launch_times(self.h, n.value, d, ms, ctypes.byref(n)))
+        return [(d[k], ms[k]) for k in range(n.value)]
+
     def set_pass_lag_buffer(self, enable: bool):
         """Third buffer of runs in passes (default on): the last pass also writes the
         state one step before the latest; off, runs end on a single step instead.  Results

@@ -3,8 +3,9 @@ forward map: 512x512 BGK, tau = 0.6, 500 steps, checkpoints every 50, band-limit
 initial velocity (|k| <= 8, amplitude 0.02, seed 1), target from seed 2.
 
 FP64 adjoint: <grad, d> within 1e-6 relative of the FD quotient on 16 random directions of
-the initial field, and for tau (measured on MI355X: 1.6e-9 and 2.7e-11).  FP32 forward +
-adjoint: within the stated 5e-4 (measured 1.7e-5 and 6.5e-5)."""
+the initial field, and for tau (measured on MI355X: 1.6e-9 and 2.7e-11).  FP32 forward and
+checkpoints with the FP64 adjoint state: within 6e-6, twice the measured 2.9e-6 (tau:
+1.5e-6); that is the deviation of the FP32 trajectory's gradient from the FP64 one."""
 from __future__ import annotations
 
 import numpy as np
@@ -33,7 +34,7 @@ def fd_loss(eng64, rho0, ux0, uy0, tau, uxt, uyt):
     return eng64.forward(rho0, ux0.contiguous(), uy0.contiguous(), 1.0 / tau, uxt, uyt)
 
 
-@pytest.mark.parametrize("storage,tol", [("f64", 1e-6), ("f32s", 5e-4)])
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-6), ("f32s", 6e-6)])
 def test_adjoint_matches_central_fd(P, gpu, storage, tol):
     import torch
     print(f"case adjoint {storage} {N}^2 x {STEPS}", flush=True)

@@ -74,6 +74,11 @@ def test_batch_autotune_and_reductions(P, gpu):
         r, u, v = b.macros(i)
         assert red[i, P._abi.RED_MASS] == pytest.approx(r.sum(), rel=1e-12)
         assert red[i, P._abi.RED_UMAX] == pytest.approx(np.hypot(u, v).max(), rel=1e-12)
+        assert red[i, P._abi.RED_KE] == pytest.approx(0.5 * np.sum(r * (u * u + v * v)),
+                                                      rel=1e-12)
+        assert red[i, P._abi.RED_U2] == pytest.approx(np.sum(u * u + v * v), rel=1e-12)
+        assert red[i, P._abi.RED_NONFINITE] == 0
+    np.testing.assert_array_equal(b.reduce(), red)  # deterministic
     with pytest.raises(P.PlbmError):
         b.set_launch(48, 1, 0, 256)
 

@@ -121,8 +121,11 @@ def test_upload_download_roundtrip(P, gpu):
         lat.close()
 
 
-def test_reductions(P, gpu, oracle):
-    nx, ny = 70, 45
+# ragged shapes, a single row or column, and a grid larger than one sweep of the 1024
+# reduction workgroups (several cells per thread)
+@pytest.mark.parametrize("nx,ny", [(70, 45), (5, 3), (1, 7), (300, 1), (1000, 530)])
+def test_reductions(P, gpu, oracle, nx, ny):
+    print(f"case reductions {nx}x{ny}", flush=True)
     rho, ux, uy = random_fields(nx, ny, 11)
     lat = P.Lattice(nx, ny, 2, "f64")
     lat.set_params(1.5, 0.25)
