@@ -204,6 +204,17 @@ int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
                 double out[PLBM_NRED]);
 int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *uya,
                     double out[PLBM_NRED]);
+/* plbm_reduce_dev without any allocation (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and the same result bits as
+ * plbm_reduce_dev, but out_dev is a DEVICE array of PLBM_NRED doubles, written
+ * asynchronously on the lattice's stream (no host synchronisation), and the partial sums
+ * live in a static device array of 1024 x PLBM_NRED doubles (64 KiB per GPU, held until
+ * the process ends).  Grids of any size fit it (at most 1024 reduction workgroups; a
+ * larger count is refused with PLBM_ERR_ARG).  All calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int plbm_reduce_static(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                       double *out_dev);
 
 int plbm_sync(plbm_lattice *h);
 const char *plbm_last_error(void);

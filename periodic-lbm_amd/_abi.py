@@ -69,6 +69,7 @@ SIGNATURES = {
     "plbm_macros_dev": (_i, [_vp, _i, _vp, _vp, _vp]),
     "plbm_reduce": (_i, [_vp, _i, _dp, _dp, _dp]),
     "plbm_reduce_dev": (_i, [_vp, _i, _vp, _vp, _dp]),
+    "plbm_reduce_static": (_i, [_vp, _i, _vp, _vp, _vp]),
     "plbm_sync": (_i, [_vp]),
     "plbm_last_error": (ctypes.c_char_p, []),
     "plbm_field_taylor_green": (_i, [_i, _i, _d, _d, _d, _d, _d, _dp, _dp, _dp]),

@@ -309,6 +309,42 @@ static int launch_reduce(plbm_lattice *h, int buf, const double *uxa, const doub
     return PLBM_OK;
 }
 
+// Scratch of plbm_reduce_static, which allocates nothing: the partial sums of the
+// reduction workgroups (kStaticReduceParts x PLBM_NRED doubles, 64 KiB), one copy per
+// device for the life of the process.  Host code must not pass `scratch` itself to a
+// kernel (on the host it names a stand-in, not device memory): its device address is
+// looked up once per device with hipGetSymbolAddress and kept in g_static_parts.  Every
+// call on a device shares that device's copy; each call overwrites every slot it reads
+// (one per workgroup it launches) before reading it, so nothing needs clearing, and calls
+// that overlap on one device race.
+constexpr int kStaticReduceParts = kReduceBlocks;
+constexpr int kStaticMaxDevices = 64;
+static __device__ __align__(256) double scratch[kStaticReduceParts * PLBM_NRED];
+static double *g_static_parts[kStaticMaxDevices] = {};
+static std::mutex g_static_mu;
+
+static int static_reduce_scratch(int dev, double **parts) {
+    PLBM_REQUIRE(dev >= 0 && dev < kStaticMaxDevices, PLBM_ERR_ARG,
+                 "device ordinal out of range for the static scratch table");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_parts[dev]) {
+        int cur = -1;
+        PLBM_HIP(hipGetDevice(&cur));
+        if (cur != dev) PLBM_HIP(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return PLBM_ERR_HIP;
+        }
+        g_static_parts[dev] = static_cast<double *>(p);
+    }
+    *parts = g_static_parts[dev];
+    return PLBM_OK;
+}
+
 static dim3 cell_grid(const plbm_lattice *h) {
     return dim3((unsigned)((h->ny + 255) / 256), (unsigned)h->nx);
 }
@@ -756,6 +792,22 @@ int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *u
     PLBM_HIP(hipMemcpyAsync(out, h->d_red, PLBM_NRED * 8, hipMemcpyDeviceToHost, h->stream));
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
+}
+
+int plbm_reduce_static(plbm_lattice *h, int buf, const double *uxa, const double *uya,
+                       double *out_dev) {
+    PLBM_TRY(check_buf(h, buf));
+    PLBM_REQUIRE(out_dev, PLBM_ERR_ARG, "null output");
+    PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
+                 "uxa and uya must both be given or both be NULL");
+    const int nb = reduce_blocks(h);
+    PLBM_REQUIRE(nb <= kStaticReduceParts, PLBM_ERR_ARG,
+                 "plbm_reduce_static: " + std::to_string(nb) +
+                     " partial sums exceed the static scratch of " +
+                     std::to_string(kStaticReduceParts));
+    double *parts = nullptr;
+    PLBM_TRY(static_reduce_scratch(h->device, &parts));
+    return launch_reduce(h, buf, uxa, uya, parts, out_dev);
 }
 
 int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
