@@ -31,6 +31,11 @@ BYTES_PER_LU = {"f64": 144, "f64s": 144, "f32s": 72, "f16s": 36}
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line stays the only stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +161,7 @@ def gpu_leg(args):
                             "launch": {"block": block, "xcd_remap": remap}}
         lat.close()
         torch.cuda.empty_cache()
+        progress(f"trt {storage}: {mlups:.0f} MLUPS")
 
     # accuracy of the shifted forms vs the FP64 run (reference arithmetic) at acc-steps
     if args.acc_steps > 0:
@@ -163,18 +169,23 @@ def gpu_leg(args):
         for storage, v in acc.items():
             results[storage].update(v)
         torch.cuda.empty_cache()
+        progress("trt accuracy")
 
     copy_gbs = P.copy_bandwidth(4 << 30, 20)
-    extras = {
-        "config1": config1_leg(P, torch, stream),
-        "config2": config2_leg(P, torch, stream) if n == N else None,
-        "collisions": None if args.no_rr else collisions_leg(P, torch, args, stream, omega,
-                                                             (d_rho, d_ux, d_uy)),
-        "schemes": schemes_leg(P, torch, args, stream, (d_rho, d_ux, d_uy))
-        if args.next_steps > 0 else None,
-        "batched_sweep": batch_leg(P, torch, args) if args.batch_steps > 0 else None,
-        "adjoint": None if args.no_adjoint else adjoint_leg(P, torch, args),
-    }
+    legs = [
+        ("config1", lambda: config1_leg(P, torch, stream)),
+        ("config2", lambda: config2_leg(P, torch, stream) if n == N else None),
+        ("collisions", lambda: None if args.no_rr else collisions_leg(
+            P, torch, args, stream, omega, (d_rho, d_ux, d_uy))),
+        ("schemes", lambda: schemes_leg(P, torch, args, stream, (d_rho, d_ux, d_uy))
+         if args.next_steps > 0 else None),
+        ("batched_sweep", lambda: batch_leg(P, torch, args) if args.batch_steps > 0 else None),
+        ("adjoint", lambda: None if args.no_adjoint else adjoint_leg(P, torch, args)),
+    ]
+    extras = {}
+    for name, leg in legs:
+        extras[name] = leg()
+        progress(f"{name} done")
     return results, copy_gbs, dev_name, extras
 
 
@@ -604,6 +615,7 @@ def main():
     os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
     results, copy_gbs, dev_name, extras = gpu_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
+    progress("cpu baseline done")
 
     f64 = results["f64"]
     n = args.n

@@ -144,6 +144,13 @@ int plbm_copy(plbm_lattice *h, int src, int dst);
 #define PLBM_STREAM_FVM_BARDOW 1
 #define PLBM_STREAM_FDM_BARDOW 2
 #define PLBM_STREAM_FDM_SOFONEA 3
+/* stream_fdm_bardow with the stencils the reference selects at compile time
+ * (src/fvm_bardow.F90:591-669): -DFDM_WLS, -DFDM_WLS_GAUSS_V1, -DFDM_WLS_GAUSS_V2,
+ * -DFDM_ISO.  Bit-identical in FP64 to the reference built with that option. */
+#define PLBM_STREAM_FDM_BARDOW_WLS 4
+#define PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V1 5
+#define PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V2 6
+#define PLBM_STREAM_FDM_BARDOW_ISO 7
 int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
 /* perform_step (src/fvm_bardow.F90:307-320) with grid%streaming = streaming operator `op`
  * and grid%collision = collide_<model>, fused into one pass: src -> stream op -> collide ->
@@ -164,7 +171,8 @@ int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *iold, int *
 int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps);
 int plbm_get_op_steps_per_pass(const plbm_lattice *h, int op, int model, int *steps);
 
-/* DUGKS, src/periodic_dugks.F90 (default build, without -DDUGKS), FP64 storage only
+/* DUGKS, src/periodic_dugks.F90 (default build, without -DDUGKS; the -DDUGKS form is
+ * plbm_dugks_stream_faces / plbm_dugks_step_faces below), FP64 storage only
  * (PLBM_ERR_STATE otherwise).  omega_full is grid%omega, omega_half the reference's
  * 1/(tau/dt + 1/2) (:56).
  *   plbm_dugks_collide  dugks_collide (:46-77): dst = BGK(src, omega_full),
@@ -188,6 +196,18 @@ int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, double dt,
  * passes of up to this many steps over device memory (1 = the collide + flux pair every
  * step; 2..3 = passes; 0 = chosen from the grid size).  Bit-identical for every value. */
 int plbm_dugks_set_steps_per_pass(plbm_lattice *h, int steps);
+/* DUGKS as the reference builds it with -DDUGKS (src/periodic_dugks.F90:68,199,267,308):
+ * the stream relaxes the interpolated face values of every face towards the face's own
+ * equilibrium (update_ew / update_ns) with rate omega_face = 1/(4 tau/dt + 1) before the
+ * flux balance, and dugks_collide's half collision runs at 0.75/(tau/dt + 1/2), which the
+ * caller passes as omega_half to plbm_dugks_collide / plbm_dugks_step_faces.
+ *   plbm_dugks_stream_faces  dugks_stream of that build, src -> dst (q = 1..8).
+ *   plbm_dugks_step_faces    nsteps x perform_dugks_step of that build, with the slot
+ *                            rules of plbm_dugks_step (one collide + flux pair per step).
+ * FP64 storage only.  Asynchronous. */
+int plbm_dugks_stream_faces(plbm_lattice *h, int src, int dst, double dt, double omega_face);
+int plbm_dugks_step_faces(plbm_lattice *h, int iold, int inew, int nsteps, double dt,
+                          double omega_full, double omega_half, double omega_face);
 int plbm_dugks_get_steps_per_pass(const plbm_lattice *h, int *steps);
 
 /* ---------------------------------------------------------------- read-back */

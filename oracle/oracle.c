@@ -130,10 +130,23 @@ void orc_stream_fvm_bardow(int nx, int ny, int ld, const double *fold, double *f
     }
 }
 
-/* stream_fdm_bardow, src/fvm_bardow.F90:526-685, default stencil (no FDM_* option):
- * central first and second differences, Lax-Wendroff update. */
-void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt) {
+/* stream_fdm_bardow, src/fvm_bardow.F90:526-685: first and second differences of the
+ * 9-point neighbourhood, Lax-Wendroff update.  variant selects the stencil the reference
+ * builds with (src/fvm_bardow.F90:591-669): 0 default (central differences), 1 -DFDM_WLS,
+ * 2 -DFDM_WLS_GAUSS_V1, 3 -DFDM_WLS_GAUSS_V2, 4 -DFDM_ISO.  Every sum is grouped left to
+ * right, as the Fortran source evaluates it. */
+void orc_stream_fdm_bardow_v(int variant, int nx, int ny, int ld, const double *fold,
+                             double *fnew, double dt) {
     const double p2 = 0.5;
+    const double one_sixth = 1.0 / 6.0, one_third = 1.0 / 3.0, two_thirds = 2.0 / 3.0;
+    const double five_sixths = 10.0 / 12.0, one_twelth = 1.0 / 12.0;
+    const int v1 = variant == 2;
+    const double p1s = v1 ? 0.2880584423829145035434 : 0.3934930210807994210853,
+                 p1d = v1 ? 0.1059707788085427065949 : 0.05325348945960039354075,
+                 p2c = v1 ? -1.152233769531658458263 : -1.573972084323197018207,
+                 p2d1 = v1 ? 0.5761168847658292291314 : 0.7869860421615988421706,
+                 p2d2 = v1 ? -0.4238831152341712149578 : -0.2130139578384016019186,
+                 p2d = v1 ? 0.2119415576170855242122 : 0.1065069789192007732037;
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y) fnew[IDX(0, x, y)] = fold[IDX(0, x, y)];
     for (int q = 1; q < 9; ++q) {
@@ -149,14 +162,58 @@ void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *f
                              fs = fold[IDX(q, x, ym)], fne = fold[IDX(q, xp, yp)],
                              fnw = fold[IDX(q, xm, yp)], fsw = fold[IDX(q, xm, ym)],
                              fse = fold[IDX(q, xp, ym)];
-                const double dfx = p2 * (fe - fw), dfy = p2 * (fn - fs);
-                const double dfxx = (fe - 2.0 * fc) + fw, dfyy = (fn - 2.0 * fc) + fs;
-                const double dfxy = 0.25 * (((fne - fse) - fnw) + fsw);
+                double dfx, dfy, dfxx, dfyy, dfxy;
+                switch (variant) {
+                case 1: /* FDM_WLS */
+                    dfx = one_sixth * (((fne - fnw) + (fe - fw)) + (fse - fsw));
+                    dfy = one_sixth * (((fne - fse) + (fn - fs)) + (fnw - fsw));
+                    dfxx = (one_third * ((fne - 2.0 * fn) + fnw) +
+                            one_third * ((fe - 2.0 * fc) + fw)) +
+                           one_third * ((fse - 2.0 * fs) + fsw);
+                    dfyy = (one_third * ((fne - 2.0 * fe) + fse) +
+                            one_third * ((fn - 2.0 * fc) + fs)) +
+                           one_third * ((fnw - 2.0 * fw) + fsw);
+                    dfxy = 0.25 * (((fne - fnw) + fsw) - fse);
+                    break;
+                case 2: /* FDM_WLS_GAUSS_V1 */
+                case 3: /* FDM_WLS_GAUSS_V2 */
+                    dfx = (p1s * (fe - fw) + p1d * (fne - fnw)) + p1d * (fse - fsw);
+                    dfy = (p1s * (fn - fs) + p1d * (fne - fse)) + p1d * (fnw - fsw);
+                    dfxx = ((p2c * fc + p2d1 * (fe + fw)) + p2d2 * (fn + fs)) +
+                           p2d * (((fne + fnw) + fsw) + fse);
+                    dfyy = ((p2c * fc + p2d2 * (fe + fw)) + p2d1 * (fn + fs)) +
+                           p2d * (((fne + fnw) + fsw) + fse);
+                    dfxy = 0.25 * (((fne - fnw) + fsw) - fse);
+                    break;
+                case 4: /* FDM_ISO */
+                    dfx = p2 * ((one_sixth * (fne - fnw) + two_thirds * (fe - fw)) +
+                                one_sixth * (fse - fsw));
+                    dfy = p2 * ((one_sixth * (fne - fse) + two_thirds * (fn - fs)) +
+                                one_sixth * (fnw - fsw));
+                    dfxx = (one_twelth * ((fne - 2.0 * fn) + fnw) +
+                            five_sixths * ((fe - 2.0 * fc) + fw)) +
+                           one_twelth * ((fse - 2.0 * fs) + fsw);
+                    dfyy = (one_twelth * ((fne - 2.0 * fe) + fse) +
+                            five_sixths * ((fn - 2.0 * fc) + fs)) +
+                           one_twelth * ((fnw - 2.0 * fw) + fsw);
+                    dfxy = 0.25 * (((fne - fse) - fnw) + fsw);
+                    break;
+                default:
+                    dfx = p2 * (fe - fw);
+                    dfy = p2 * (fn - fs);
+                    dfxx = (fe - 2.0 * fc) + fw;
+                    dfyy = (fn - 2.0 * fc) + fs;
+                    dfxy = 0.25 * (((fne - fse) - fnw) + fsw);
+                }
                 fnew[IDX(q, x, y)] =
                     ((fc - cxq * dfx) - cyq * dfy) + ((cxxq * dfxx + cxyq * dfxy) + cyyq * dfyy);
             }
         }
     }
+}
+
+void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt) {
+    orc_stream_fdm_bardow_v(0, nx, ny, ld, fold, fnew, dt);
 }
 
 /* stream_fdm_sofonea, src/fvm_bardow.F90:702-889: per direction, a 1-D Lax-Wendroff
@@ -284,6 +341,79 @@ void orc_dugks_stream(int nx, int ny, int ld, const double *ft, double *fp, doub
                                    (p8 * cxq) * (((fse + fe) - fsw) - fw);
                 const size_t i = IDX(q, x, y);
                 fp[i] = (fp[i] - cxq * (cfe - cfw)) - cyq * (cfn - cfs);
+            }
+        }
+    }
+}
+
+/* update_ew / update_ns of the -DDUGKS build (src/periodic_dugks.F90:308-420): a BGK
+ * relaxation of the interpolated face values of one face towards the face's own
+ * equilibrium, with rate omega; the east/west form updates q = 1, 3, 5, 7, 6, 8 and the
+ * north/south form q = 2, 4, 5, 7, 6, 8 (the others carry no flux across that face). */
+static void orc_dugks_face_collide(double f[9], double omega, int ew) {
+    const double one_third = 1.0 / 3.0, w0 = 4.0 / 9.0, ws = 1.0 / 9.0, wd = 1.0 / 36.0;
+    const double omegabar = 1.0 - omega;
+    const double omega_ws = (3.0 * omega) * ws, omega_wd = (3.0 * omega) * wd;
+    (void)w0;
+    const double rho = (((f[5] + f[7]) + (f[6] + f[8])) + ((f[1] + f[3]) + (f[2] + f[4]))) + f[0];
+    const double invrho = 1.0 / rho;
+    const double ux = invrho * (((f[5] - f[7]) + (f[8] - f[6])) + (f[1] - f[3]));
+    const double uy = invrho * (((f[5] - f[7]) + (f[6] - f[8])) + (f[2] - f[4]));
+    const double indp = one_third - 0.5 * (ux * ux + uy * uy);
+    if (ew) {
+        const double vel_trm_13 = indp + (1.5 * ux) * ux;
+        f[1] = omegabar * f[1] + (omega_ws * rho) * (vel_trm_13 + ux);
+        f[3] = omegabar * f[3] + (omega_ws * rho) * (vel_trm_13 - ux);
+    } else {
+        const double vel_trm_24 = indp + (1.5 * uy) * uy;
+        f[2] = omegabar * f[2] + (omega_ws * rho) * (vel_trm_24 + uy);
+        f[4] = omegabar * f[4] + (omega_ws * rho) * (vel_trm_24 - uy);
+    }
+    const double velxpy = ux + uy, vel_trm_57 = indp + (1.5 * velxpy) * velxpy;
+    f[5] = omegabar * f[5] + (omega_wd * rho) * (vel_trm_57 + velxpy);
+    f[7] = omegabar * f[7] + (omega_wd * rho) * (vel_trm_57 - velxpy);
+    const double velxmy = ux - uy, vel_trm_68 = indp + (1.5 * velxmy) * velxmy;
+    f[6] = omegabar * f[6] + (omega_wd * rho) * (vel_trm_68 - velxmy);
+    f[8] = omegabar * f[8] + (omega_wd * rho) * (vel_trm_68 + velxmy);
+}
+
+/* kernel_stream of the -DDUGKS build (src/periodic_dugks.F90:190-299): the face values of
+ * all nine directions (q = 0 included: the faces' moments need them), a face collision
+ * on each of the four faces (update_ew on west and east, update_ns on north and south,
+ * rate omega_face = 1/(4 tau/dt + 1)), then the flux balance for q = 1..8. */
+void orc_dugks_stream_face(int nx, int ny, int ld, const double *ft, double *fp, double dt,
+                           double omega_face) {
+    const double p2 = 0.5, p8 = 0.125;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x) {
+        const int xp = (x + 1) % nx, xm = (x + nx - 1) % nx;
+        for (int y = 0; y < ny; ++y) {
+            const int yp = (y + 1) % ny, ym = (y + ny - 1) % ny;
+            double cfw[9], cfn[9], cfe[9], cfs[9];
+            for (int q = 0; q < 9; ++q) {
+                const double cxq = dt * ORC_CX[q], cyq = dt * ORC_CY[q];
+                const double fc = ft[IDX(q, x, y)], fe = ft[IDX(q, xp, y)],
+                             fn = ft[IDX(q, x, yp)], fw = ft[IDX(q, xm, y)],
+                             fs = ft[IDX(q, x, ym)], fne = ft[IDX(q, xp, yp)],
+                             fnw = ft[IDX(q, xm, yp)], fsw = ft[IDX(q, xm, ym)],
+                             fse = ft[IDX(q, xp, ym)];
+                cfw[q] = (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) -
+                         (p8 * cyq) * (((fnw + fn) - fsw) - fs);
+                cfn[q] = (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) -
+                         (p8 * cxq) * (((fne + fe) - fnw) - fw);
+                cfe[q] = (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) -
+                         (p8 * cyq) * (((fne + fn) - fse) - fs);
+                cfs[q] = (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) -
+                         (p8 * cxq) * (((fse + fe) - fsw) - fw);
+            }
+            orc_dugks_face_collide(cfw, omega_face, 1);
+            orc_dugks_face_collide(cfe, omega_face, 1);
+            orc_dugks_face_collide(cfn, omega_face, 0);
+            orc_dugks_face_collide(cfs, omega_face, 0);
+            for (int q = 1; q < 9; ++q) {
+                const double cxq = dt * ORC_CX[q], cyq = dt * ORC_CY[q];
+                const size_t i = IDX(q, x, y);
+                fp[i] = (fp[i] - cxq * (cfe[q] - cfw[q])) - cyq * (cfn[q] - cfs[q]);
             }
         }
     }

@@ -37,11 +37,17 @@ void orc_stream(int nx, int ny, int ld, const double *fsrc, double *fdst);
 /* Finite-volume / finite-difference streaming operators (src/fvm_bardow.F90:393-893) */
 void orc_stream_fvm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
 void orc_stream_fdm_bardow(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
+/* variant: 0 default, 1 FDM_WLS, 2 FDM_WLS_GAUSS_V1, 3 FDM_WLS_GAUSS_V2, 4 FDM_ISO */
+void orc_stream_fdm_bardow_v(int variant, int nx, int ny, int ld, const double *fold,
+                             double *fnew, double dt);
 void orc_stream_fdm_sofonea(int nx, int ny, int ld, const double *fold, double *fnew, double dt);
 /* DUGKS, src/periodic_dugks.F90 (default build) */
 void orc_dugks_collide(int nx, int ny, int ld, double *fold, double *fnew, double omega_full,
                        double omega_half);
 void orc_dugks_stream(int nx, int ny, int ld, const double *ft, double *fp, double dt);
+/* the -DDUGKS build's stream: face collisions with rate omega_face before the fluxes */
+void orc_dugks_stream_face(int nx, int ny, int ld, const double *ft, double *fp, double dt,
+                           double omega_face);
 void orc_collide_bgk(int nx, int ny, int ld, double *f, double omega);
 /* collide_bgk of the -DSPLIT build (bgk_kernel_cache, src/collision_bgk.F90:84-176) */
 void orc_collide_bgk_split(int nx, int ny, int ld, double *f, double omega);

@@ -74,9 +74,11 @@ class Oracle:
         L.orc_stream.argtypes = [_i, _i, _i, _dp, _dp]
         for name in ("orc_stream_fvm_bardow", "orc_stream_fdm_bardow", "orc_stream_fdm_sofonea"):
             getattr(L, name).argtypes = [_i, _i, _i, _dp, _dp, _d]
+        L.orc_stream_fdm_bardow_v.argtypes = [_i, _i, _i, _i, _dp, _dp, _d]
         L.orc_collide_bgk.argtypes = [_i, _i, _i, _dp, _d]
         L.orc_dugks_collide.argtypes = [_i, _i, _i, _dp, _dp, _d, _d]
         L.orc_dugks_stream.argtypes = [_i, _i, _i, _dp, _dp, _d]
+        L.orc_dugks_stream_face.argtypes = [_i, _i, _i, _dp, _dp, _d, _d]
         L.orc_collide_trt.argtypes = [_i, _i, _i, _dp, _d, _d]
         L.orc_collide_rr.argtypes = [_i, _i, _i, _dp, _d]
         L.orc_collide.argtypes = [_i, _i, _i, _i, _dp, _d, _d]
@@ -118,10 +120,18 @@ class Oracle:
     def stream(self, nx, ny, fsrc, fdst):
         self.lib.orc_stream(nx, ny, fsrc.shape[0], _ptr(fsrc), _ptr(fdst))
 
+    # stream_fdm_bardow's compile-time stencils (src/fvm_bardow.F90:591-669)
+    FDM_VARIANTS = {"fdm_bardow_wls": 1, "fdm_bardow_wls_gauss_v1": 2,
+                    "fdm_bardow_wls_gauss_v2": 3, "fdm_bardow_iso": 4}
+
     def stream_op(self, op, nx, ny, fsrc, fdst, dt=1.0):
-        """op: "exact" | "fvm_bardow" | "fdm_bardow" | "fdm_sofonea"."""
+        """op: "exact" | "fvm_bardow" | "fdm_bardow" | "fdm_sofonea" | one of
+        FDM_VARIANTS."""
         if op == "exact":
             return self.stream(nx, ny, fsrc, fdst)
+        if op in self.FDM_VARIANTS:
+            return self.lib.orc_stream_fdm_bardow_v(self.FDM_VARIANTS[op], nx, ny, fsrc.shape[0],
+                                                    _ptr(fsrc), _ptr(fdst), dt)
         getattr(self.lib, "orc_stream_" + op)(nx, ny, fsrc.shape[0], _ptr(fsrc), _ptr(fdst), dt)
 
     def collide(self, model, nx, ny, f, omega, trt_magic=0.25):
@@ -139,6 +149,11 @@ class Oracle:
     def dugks_stream(self, nx, ny, ft, fp, dt):
         """dugks_stream: fp -= flux balance of the face values of ft."""
         self.lib.orc_dugks_stream(nx, ny, ft.shape[0], _ptr(ft), _ptr(fp), dt)
+
+    def dugks_stream_face(self, nx, ny, ft, fp, dt, omega_face):
+        """dugks_stream of the -DDUGKS build: face collisions (rate omega_face) before the
+        flux balance."""
+        self.lib.orc_dugks_stream_face(nx, ny, ft.shape[0], _ptr(ft), _ptr(fp), dt, omega_face)
 
     def dugks_run(self, nx, ny, fa, fb, nsteps, dt, omega_full, omega_half):
         """perform_dugks_step x nsteps on (fa = iold, fb = inew), with the reference's

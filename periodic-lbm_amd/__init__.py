@@ -33,9 +33,13 @@ __all__ += ["VortexCase", "dugks_collide", "dugks_stream", "perform_dugks_step",
             "stream_fdm_bardow", "stream_fdm_sofonea", "stream_fvm_bardow", "tau_over_dt"]
 
 __all__ += ["Batch", "Case", "sweep_cases"]
-from .lattice import collide_bgk_split, perform_dugks_steps, perform_steps  # noqa: E402
+from .lattice import (  # noqa: E402
+    FDM_BARDOW_STENCILS, collide_bgk_split, perform_dugks_steps, perform_steps,
+    set_dugks_face_collisions, set_fdm_bardow_stencil,
+)
 
-__all__ += ["collide_bgk_split", "perform_dugks_steps", "perform_steps"]
+__all__ += ["FDM_BARDOW_STENCILS", "collide_bgk_split", "perform_dugks_steps", "perform_steps",
+            "set_dugks_face_collisions", "set_fdm_bardow_stencil"]
 from .adjoint import Adjoint, band_limited_field, velocity_misfit  # noqa: E402
 
 __all__ += ["Adjoint", "band_limited_field", "velocity_misfit"]

@@ -126,9 +126,13 @@ SIGNATURES = {
     "plbm_dugks_step": (_i, [_vp, _i, _i, _i, _d, _d, _d]),
     "plbm_dugks_set_steps_per_pass": (_i, [_vp, _i]),
     "plbm_dugks_get_steps_per_pass": (_i, [_vp, _ip]),
+    "plbm_dugks_stream_faces": (_i, [_vp, _i, _i, _d, _d]),
+    "plbm_dugks_step_faces": (_i, [_vp, _i, _i, _i, _d, _d, _d, _d]),
     "plbm_field_barotropic_vortex": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _d, _dp, _dp, _dp]),
 }
-STREAM_OPS = {"exact": 0, "fvm_bardow": 1, "fdm_bardow": 2, "fdm_sofonea": 3}
+STREAM_OPS = {"exact": 0, "fvm_bardow": 1, "fdm_bardow": 2, "fdm_sofonea": 3,
+              "fdm_bardow_wls": 4, "fdm_bardow_wls_gauss_v1": 5, "fdm_bardow_wls_gauss_v2": 6,
+              "fdm_bardow_iso": 7}
 
 _LIB = None
 

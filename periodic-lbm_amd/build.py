@@ -29,8 +29,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("PLBM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip", "dugks.hip",
-               "multistep.hip"]
+HIP_SOURCES = ["lattice.hip", "batch.hip", "adjoint.hip", "streaming_ops.hip",
+               "streaming_ops_fdm.hip", "dugks.hip", "multistep.hip"]
 CXX_SOURCES = ["fields.cpp", "sim_plugin.cpp"]
 # -ffp-contract=off: the FP64 reference path must not contract a*b+c into an FMA (the
 # reference is built without FMA); kernels that want FMA enable it locally by pragma.
@@ -91,17 +91,22 @@ def build_hip(verbose: bool = True) -> Path:
 FORTRAN_DEFINES = os.environ.get("PLBM_FORTRAN_DEFINES", "").split()
 
 
-def build_fortran(verbose: bool = True) -> Path:
-    """Compile the Fortran module shim (ISO_C_BINDING over libplbm.so)."""
-    mod = FBUILD / "mod"
+def build_fortran(verbose: bool = True, outdir: Path | None = None,
+                  defines: list[str] | None = None) -> Path:
+    """Compile the Fortran module shim (ISO_C_BINDING over libplbm.so) into outdir
+    (default fortran/_build) with the given preprocessor options (default
+    PLBM_FORTRAN_DEFINES)."""
+    outdir = FBUILD if outdir is None else outdir
+    defines = FORTRAN_DEFINES if defines is None else defines
+    mod = outdir / "mod"
     mod.mkdir(parents=True, exist_ok=True)
     objs = []
     for s in FORTRAN_SOURCES:
-        o = FBUILD / (Path(s).stem + ".o")
-        _run([FLANG, "-O2", "-fPIC", "-fopenmp", *FORTRAN_DEFINES, "-module-dir", mod,
+        o = outdir / (Path(s).stem + ".o")
+        _run([FLANG, "-O2", "-fPIC", "-fopenmp", *defines, "-module-dir", mod,
               f"-I{mod}", "-c", FSRC / s, "-o", o])
         objs.append(o)
-    lib = FBUILD / "libplbm_fortran.a"
+    lib = outdir / "libplbm_fortran.a"
     if lib.exists():
         lib.unlink()
     _run(["ar", "rcs", lib, *objs])
@@ -110,22 +115,30 @@ def build_fortran(verbose: bool = True) -> Path:
     return lib
 
 
-def link_fortran_program(src: Path, out: Path, moddir: Path | None = None) -> Path:
+def link_fortran_program(src: Path, out: Path, moddir: Path | None = None,
+                         shim: Path | None = None) -> Path:
     """Compile a Fortran program that `use`s the reference module names against the
-    shim and libplbm.so.  The internal-procedure logger of the reference app needs an
-    executable stack for its trampoline."""
-    mod = FBUILD / "mod"
+    shim (default fortran/_build) and libplbm.so.  The internal-procedure logger of the
+    reference app needs an executable stack for its trampoline."""
+    shim = FBUILD if shim is None else shim
+    mod = shim / "mod"
     out.parent.mkdir(parents=True, exist_ok=True)
     pmod = moddir or (out.parent / "mod")
     pmod.mkdir(parents=True, exist_ok=True)
     _run([FLANG, "-O2", "-fopenmp", f"-I{mod}", "-module-dir", pmod, src,
-          FBUILD / "libplbm_fortran.a", f"-L{PKG}", "-lplbm", f"-Wl,-rpath,{PKG}",
+          shim / "libplbm_fortran.a", f"-L{PKG}", "-lplbm", f"-Wl,-rpath,{PKG}",
           "-Wl,-rpath,$ORIGIN/../..", "-Wl,-z,execstack", "-o", out])
     return out
 
 
+# The shim compiled with the reference's -DDUGKS (DUGKS with face collisions), for the
+# DUGKS test program built against it.
+FBUILD_DUGKS = FSRC / "_build_dugks"
+
+
 def build_fortran_tests(verbose: bool = True) -> None:
-    """The Fortran test programs under tests/fortran, linked against the shim."""
+    """The Fortran test programs under tests/fortran, linked against the shim; and
+    dugks_check once more against the shim compiled with -DDUGKS (dugks_check_faces)."""
     out = PKG / "_build" / "tests"
     for name in ("shim_check", "outputs_check", "dugks_check"):
         src = ROOT / "tests" / "fortran" / f"{name}.f90"
@@ -133,6 +146,13 @@ def build_fortran_tests(verbose: bool = True) -> None:
             link_fortran_program(src, out / name, moddir=out / f"mod_{name}")
             if verbose:
                 print(f"[build] {(out / name).relative_to(ROOT)}")
+    src = ROOT / "tests" / "fortran" / "dugks_check.f90"
+    if src.exists():
+        build_fortran(False, FBUILD_DUGKS, [*FORTRAN_DEFINES, "-DDUGKS"])
+        link_fortran_program(src, out / "dugks_check_faces", moddir=out / "mod_dugks_faces",
+                             shim=FBUILD_DUGKS)
+        if verbose:
+            print(f"[build] {(out / 'dugks_check_faces').relative_to(ROOT)}")
 
 
 def build_app(ref_root: Path, verbose: bool = True) -> None:

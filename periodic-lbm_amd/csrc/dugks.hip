@@ -1,5 +1,5 @@
-// dugks.hip — the discrete unified gas-kinetic scheme of src/periodic_dugks.F90 (default
-// build, no -DDUGKS) on FP64 storage:
+// dugks.hip — the discrete unified gas-kinetic scheme of src/periodic_dugks.F90 on FP64
+// storage; the default build (no -DDUGKS):
 //
 //   dugks_collide  (:46-77)   fnew = BGK(fold, grid%omega)           (the copy + collision)
 //                             fold = BGK(fold, 1/(tau/dt + 1/2))      (in place: \bar f^+)
@@ -22,6 +22,9 @@
 // steps per sweep, 144 bytes per cell per pass), and the two slots exchange their device
 // storage where needed, so that slot contents equal the reference's buffers after the
 // same number of perform_dugks_step calls.
+// The -DDUGKS build's form (half collision at 0.75 of its rate, which the caller passes,
+// and face collisions in the stream) is plbm_dugks_stream_faces / plbm_dugks_step_faces
+// (k_dugks_flux_faces), one collide + flux pair per step.
 // All FP64 expression trees follow the reference with FMA contraction off (bit-exact).
 #include <algorithm>
 #include <cmath>
@@ -140,6 +143,89 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int j = 0; j < 3; ++j) w[q][0][j] = w[q][1][j], w[q][1][j] = w[q][2][j];
         }
+    }
+}
+
+// update_ew / update_ns of the -DDUGKS build (src/periodic_dugks.F90:308-420): BGK
+// relaxation of one face's interpolated values towards the face's own equilibrium (rate
+// omega); east/west faces update q = 1, 3, 5, 7, 6, 8, north/south faces q = 2, 4, 5, 7, 6,
+// 8 (the others carry no flux across that face).
+__device__ __forceinline__ void dugks_face_collide(double f[9], double omega, bool ew) {
+#pragma clang fp contract(off)
+    const double omegabar = 1.0 - omega;
+    const double omega_ws = (3.0 * omega) * (1.0 / 9.0), omega_wd = (3.0 * omega) * (1.0 / 36.0);
+    const double rho = (((f[5] + f[7]) + (f[6] + f[8])) + ((f[1] + f[3]) + (f[2] + f[4]))) + f[0];
+    const double invrho = 1.0 / rho;
+    const double ux = invrho * (((f[5] - f[7]) + (f[8] - f[6])) + (f[1] - f[3]));
+    const double uy = invrho * (((f[5] - f[7]) + (f[6] - f[8])) + (f[2] - f[4]));
+    const double indp = (1.0 / 3.0) - 0.5 * (ux * ux + uy * uy);
+    if (ew) {
+        const double vel_trm_13 = indp + (1.5 * ux) * ux;
+        f[1] = omegabar * f[1] + (omega_ws * rho) * (vel_trm_13 + ux);
+        f[3] = omegabar * f[3] + (omega_ws * rho) * (vel_trm_13 - ux);
+    } else {
+        const double vel_trm_24 = indp + (1.5 * uy) * uy;
+        f[2] = omegabar * f[2] + (omega_ws * rho) * (vel_trm_24 + uy);
+        f[4] = omegabar * f[4] + (omega_ws * rho) * (vel_trm_24 - uy);
+    }
+    const double velxpy = ux + uy, vel_trm_57 = indp + (1.5 * velxpy) * velxpy;
+    f[5] = omegabar * f[5] + (omega_wd * rho) * (vel_trm_57 + velxpy);
+    f[7] = omegabar * f[7] + (omega_wd * rho) * (vel_trm_57 - velxpy);
+    const double velxmy = ux - uy, vel_trm_68 = indp + (1.5 * velxmy) * velxmy;
+    f[6] = omegabar * f[6] + (omega_wd * rho) * (vel_trm_68 - velxmy);
+    f[8] = omegabar * f[8] + (omega_wd * rho) * (vel_trm_68 + velxmy);
+}
+
+// kernel_stream of the -DDUGKS build (src/periodic_dugks.F90:190-299 with the face
+// collisions at :253-262): per cell the four faces' interpolated values of all nine
+// directions (the faces' moments need them all), a face collision on each face (rate
+// om_face = 1/(4 tau/dt + 1)), then the flux balance for q = 1..8.  One thread per cell
+// reading its 3x3 neighbourhood (the face values of a cell are recomputed by each cell,
+// as the reference computes them, so that every expression is the reference's).
+// COLLIDE: fp is BGK(fo, om_full) of the cell (fo may alias fp), as in k_dugks_flux.
+template <bool COLLIDE>
+__global__ void __launch_bounds__(256)
+    k_dugks_flux_faces(const double *__restrict__ ft, const double *fo, double *fp, int nx,
+                       int ny, int ldd, double dt, double om_full, double om_face) {
+#pragma clang fp contract(off)
+    const int y = blockIdx.x * 256 + threadIdx.x, x = blockIdx.y;
+    if (y >= ny) return;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const size_t L = (size_t)ldd, cs = 9 * L;
+    const size_t c0 = (size_t)x * cs, cm = (size_t)xm * cs, cp = (size_t)xp * cs;
+    const double p2 = 0.5, p8 = 0.125;
+    double cfw[9], cfn[9], cfe[9], cfs[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const size_t o = (size_t)q * L;
+        const double cxq = dt * double(cx(q)), cyq = dt * double(cy(q));
+        const double fc = ft[c0 + o + y], fe = ft[cp + o + y], fn = ft[c0 + o + yp];
+        const double fw = ft[cm + o + y], fs = ft[c0 + o + ym], fne = ft[cp + o + yp];
+        const double fnw = ft[cm + o + yp], fsw = ft[cm + o + ym], fse = ft[cp + o + ym];
+        cfw[q] = (p2 * (fc + fw) - (p2 * cxq) * (fc - fw)) - (p8 * cyq) * (((fnw + fn) - fsw) - fs);
+        cfn[q] = (p2 * (fc + fn) - (p2 * cyq) * (fn - fc)) - (p8 * cxq) * (((fne + fe) - fnw) - fw);
+        cfe[q] = (p2 * (fc + fe) - (p2 * cxq) * (fe - fc)) - (p8 * cyq) * (((fne + fn) - fse) - fs);
+        cfs[q] = (p2 * (fc + fs) - (p2 * cyq) * (fc - fs)) - (p8 * cxq) * (((fse + fe) - fsw) - fw);
+    }
+    dugks_face_collide(cfw, om_face, true);
+    dugks_face_collide(cfe, om_face, true);
+    dugks_face_collide(cfn, om_face, false);
+    dugks_face_collide(cfs, om_face, false);
+    double g[9];
+    if constexpr (COLLIDE) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = fo[c0 + q * L + y];
+        dugks_bgk(g, om_full);
+        fp[c0 + y] = g[0];
+    } else {
+#pragma unroll
+        for (int q = 1; q < 9; ++q) g[q] = fp[c0 + q * L + y];
+    }
+#pragma unroll
+    for (int q = 1; q < 9; ++q) {
+        const double cxq = dt * double(cx(q)), cyq = dt * double(cy(q));
+        fp[c0 + (size_t)q * L + y] = (g[q] - cxq * (cfe[q] - cfw[q])) - cyq * (cfn[q] - cfs[q]);
     }
 }
 
@@ -270,9 +356,50 @@ static void launch_flux(plbm_lattice *h, const double *ft, const double *fo, dou
     else go(k_dugks_flux<COLLIDE, 1>);
 }
 
+template <bool COLLIDE>
+static void launch_flux_faces(plbm_lattice *h, const double *ft, const double *fo, double *fp,
+                              double dt, double om_full, double om_face) {
+    const dim3 grid((unsigned)dugks_nby(h), (unsigned)h->nx);
+    hipLaunchKernelGGL(k_dugks_flux_faces<COLLIDE>, grid, dim3(256), 0, h->stream, ft, fo, fp,
+                       h->nx, h->ny, h->ldd, dt, om_full, om_face);
+}
+
 }  // namespace plbm
 
 using namespace plbm;
+
+extern "C" int plbm_dugks_stream_faces(plbm_lattice *h, int src, int dst, double dt,
+                                       double omega_face) {
+    PLBM_TRY(dugks_check(h, src, dst));
+    PLBM_REQUIRE(std::isfinite(dt) && std::isfinite(omega_face), PLBM_ERR_ARG,
+                 "non-finite dt or face relaxation rate");
+    launch_flux_faces<false>(h, static_cast<const double *>(h->f[src]), nullptr,
+                             static_cast<double *>(h->f[dst]), dt, 0.0, omega_face);
+    PLBM_HIP(hipGetLastError());
+    return PLBM_OK;
+}
+
+extern "C" int plbm_dugks_step_faces(plbm_lattice *h, int iold, int inew, int nsteps,
+                                     double dt, double omega_full, double omega_half,
+                                     double omega_face) {
+    PLBM_TRY(dugks_check(h, iold, inew));
+    PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "negative step count");
+    PLBM_REQUIRE(std::isfinite(dt) && std::isfinite(omega_full) && std::isfinite(omega_half) &&
+                     std::isfinite(omega_face),
+                 PLBM_ERR_ARG, "non-finite dt or relaxation rate");
+    // each step: the half collision into the other slot, then the full collision of the
+    // cell minus the face-collided flux balance, in place over the latest state
+    double *latest = static_cast<double *>(h->f[iold]);
+    double *other = static_cast<double *>(h->f[inew]);
+    for (int s = 0; s < nsteps; ++s) {
+        launch_collide<false>(h, latest, other, nullptr, omega_full, omega_half);
+        launch_flux_faces<true>(h, other, latest, latest, dt, omega_full, omega_face);
+    }
+    PLBM_HIP(hipGetLastError());
+    const int want = (nsteps % 2) ? inew : iold;
+    if (nsteps > 0 && h->f[want] != latest) std::swap(h->f[iold], h->f[inew]);
+    return PLBM_OK;
+}
 
 extern "C" int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omega_full,
                                   double omega_half) {

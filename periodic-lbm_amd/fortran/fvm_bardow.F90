@@ -49,6 +49,29 @@ module fvm_bardow
 
    integer, parameter :: WHERE_HOST = 0, WHERE_DEVICE = 1
 
+   ! DUGKS with face collisions, chosen when this module is compiled with the reference's
+   ! -DDUGKS (src/periodic_dugks.F90:68,199,267,308): the half collision at 0.75 of its
+   ! rate, and the stream relaxes the face values at rate 1/(4 tau/dt + 1)
+#if defined(DUGKS)
+   logical, parameter :: DUGKS_FACES = .true.
+#else
+   logical, parameter :: DUGKS_FACES = .false.
+#endif
+
+   ! stream_fdm_bardow's stencil, chosen when this module is compiled, with the reference's
+   ! preprocessor options (src/fvm_bardow.F90:591-669)
+#if defined(FDM_WLS)
+   integer(c_int), parameter :: FDM_BARDOW_OP = PLBM_STREAM_FDM_BARDOW_WLS
+#elif defined(FDM_WLS_GAUSS_V1)
+   integer(c_int), parameter :: FDM_BARDOW_OP = PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V1
+#elif defined(FDM_WLS_GAUSS_V2)
+   integer(c_int), parameter :: FDM_BARDOW_OP = PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V2
+#elif defined(FDM_ISO)
+   integer(c_int), parameter :: FDM_BARDOW_OP = PLBM_STREAM_FDM_BARDOW_ISO
+#else
+   integer(c_int), parameter :: FDM_BARDOW_OP = PLBM_STREAM_FDM_BARDOW
+#endif
+
    type :: lattice_grid
 
       integer :: nx, ny
@@ -375,7 +398,7 @@ contains
       else if (associated(grid%streaming, stream_fvm_bardow)) then
          op = PLBM_STREAM_FVM_BARDOW
       else if (associated(grid%streaming, stream_fdm_bardow)) then
-         op = PLBM_STREAM_FDM_BARDOW
+         op = FDM_BARDOW_OP
       else if (associated(grid%streaming, stream_fdm_sofonea)) then
          op = PLBM_STREAM_FDM_SOFONEA
       end if
@@ -565,11 +588,12 @@ contains
       call stream_op_on_device(grid, PLBM_STREAM_FVM_BARDOW, "stream_fvm_bardow")
    end subroutine
 
-   !> stream_fdm_bardow (src/fvm_bardow.F90:511-685, default stencil): Lax-Wendroff
-   !> finite-difference streaming on the device.
+   !> stream_fdm_bardow (src/fvm_bardow.F90:511-685): Lax-Wendroff finite-difference
+   !> streaming on the device, with the stencil this module was compiled for (central
+   !> differences, or -DFDM_WLS / -DFDM_WLS_GAUSS_V1 / -DFDM_WLS_GAUSS_V2 / -DFDM_ISO).
    subroutine stream_fdm_bardow(grid)
       class(lattice_grid), intent(inout) :: grid
-      call stream_op_on_device(grid, PLBM_STREAM_FDM_BARDOW, "stream_fdm_bardow")
+      call stream_op_on_device(grid, FDM_BARDOW_OP, "stream_fdm_bardow")
    end subroutine
 
    !> stream_fdm_sofonea (src/fvm_bardow.F90:688-893): 1-D Lax-Wendroff along each
@@ -594,8 +618,7 @@ contains
       class(lattice_grid), intent(inout) :: grid
       call require_device(grid)
       call plbm_check(plbm_dugks_collide(grid%dev, int(grid%iold-1,c_int), &
-         int(grid%inew-1,c_int), grid%omega, 1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), &
-         "dugks_collide")
+         int(grid%inew-1,c_int), grid%omega, dugks_omega_half(grid)), "dugks_collide")
       call device_written(grid)
    end subroutine
 
@@ -604,10 +627,45 @@ contains
    subroutine plbm_grid_dugks_stream(grid)
       class(lattice_grid), intent(inout) :: grid
       call require_device(grid)
-      call plbm_check(plbm_dugks_stream(grid%dev, int(grid%iold-1,c_int), &
-         int(grid%inew-1,c_int), grid%dt), "dugks_stream")
+      if (DUGKS_FACES) then
+         call plbm_check(plbm_dugks_stream_faces(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), grid%dt, dugks_omega_face(grid)), "dugks_stream")
+      else
+         call plbm_check(plbm_dugks_stream(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), grid%dt), "dugks_stream")
+      end if
       call device_written(grid)
    end subroutine
+
+   !> Half-step rate of dugks_collide: 1/(tau/dt + 1/2) (src/periodic_dugks.F90:56), times
+   !> 0.75 in the -DDUGKS build (:68-71).
+   real(wp) function dugks_omega_half(grid) result(omega)
+      type(lattice_grid), intent(in) :: grid
+      omega = 1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)
+      if (DUGKS_FACES) omega = 0.75_wp*omega
+   end function
+
+   !> Face-collision rate of the -DDUGKS build's dugks_stream (src/periodic_dugks.F90:179).
+   real(wp) function dugks_omega_face(grid) result(omega)
+      type(lattice_grid), intent(in) :: grid
+      omega = 1.0_wp/(4.0_wp*plbm_tau_over_dt(grid) + 1.0_wp)
+   end function
+
+   !> The engine's fused DUGKS steps for this build (plbm_dugks_step, or
+   !> plbm_dugks_step_faces with -DDUGKS).
+   integer(c_int) function dugks_steps(grid, nsteps) result(status)
+      type(lattice_grid), intent(in) :: grid
+      integer, intent(in) :: nsteps
+      if (DUGKS_FACES) then
+         status = plbm_dugks_step_faces(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), int(nsteps,c_int), grid%dt, grid%omega, &
+            dugks_omega_half(grid), dugks_omega_face(grid))
+      else
+         status = plbm_dugks_step(grid%dev, int(grid%iold-1,c_int), &
+            int(grid%inew-1,c_int), int(nsteps,c_int), grid%dt, grid%omega, &
+            dugks_omega_half(grid))
+      end if
+   end function
 
    !> perform_dugks_step (src/periodic_dugks.F90:25-38): collision, streaming, swap.  With
    !> dugks_collide + dugks_stream in the slots both run as the engine's fused step.
@@ -618,9 +676,7 @@ contains
       if (associated(grid%collision, plbm_grid_dugks_collide) .and. &
           associated(grid%streaming, plbm_grid_dugks_stream)) then
          call require_device(grid)
-         call plbm_check(plbm_dugks_step(grid%dev, int(grid%iold-1,c_int), &
-            int(grid%inew-1,c_int), 1_c_int, grid%dt, grid%omega, &
-            1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), "perform_dugks_step")
+         call plbm_check(dugks_steps(grid, 1), "perform_dugks_step")
          call device_written(grid)
       else
          call dispatch_collision(grid)
@@ -643,9 +699,7 @@ contains
       if (associated(grid%collision, plbm_grid_dugks_collide) .and. &
           associated(grid%streaming, plbm_grid_dugks_stream)) then
          call require_device(grid)
-         call plbm_check(plbm_dugks_step(grid%dev, int(grid%iold-1,c_int), &
-            int(grid%inew-1,c_int), int(nsteps,c_int), grid%dt, grid%omega, &
-            1.0_wp/(plbm_tau_over_dt(grid) + 0.5_wp)), "plbm_perform_dugks_steps")
+         call plbm_check(dugks_steps(grid, nsteps), "plbm_perform_dugks_steps")
          call device_written(grid)
          if (mod(nsteps, 2) == 1) then
             itmp = grid%iold

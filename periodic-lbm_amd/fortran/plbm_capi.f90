@@ -17,8 +17,14 @@ module plbm_capi
    public :: plbm_equilibrium_init, plbm_step, plbm_run, plbm_stream, plbm_collide, plbm_copy
    public :: plbm_stream_op, plbm_step_op, plbm_set_steps_per_launch, plbm_run_op
    public :: plbm_dugks_collide, plbm_dugks_stream, plbm_dugks_step
+   public :: plbm_dugks_stream_faces, plbm_dugks_step_faces
    integer(c_int), parameter, public :: PLBM_STREAM_EXACT = 0, PLBM_STREAM_FVM_BARDOW = 1, &
                                         PLBM_STREAM_FDM_BARDOW = 2, PLBM_STREAM_FDM_SOFONEA = 3
+   ! stream_fdm_bardow's stencils selected at compile time (src/fvm_bardow.F90:591-669)
+   integer(c_int), parameter, public :: PLBM_STREAM_FDM_BARDOW_WLS = 4, &
+                                        PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V1 = 5, &
+                                        PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V2 = 6, &
+                                        PLBM_STREAM_FDM_BARDOW_ISO = 7
    public :: plbm_macros, plbm_reduce, plbm_sync, plbm_last_error, plbm_check
    public :: plbm_lambda_d, plbm_magic_number
 
@@ -132,6 +138,22 @@ module plbm_capi
          type(c_ptr), value :: h
          integer(c_int), value :: iold, inew, nsteps
          real(c_double), value :: dt, omega_full, omega_half
+      end function
+
+      integer(c_int) function plbm_dugks_stream_faces(h, src, dst, dt, omega_face) &
+            bind(c, name="plbm_dugks_stream_faces")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: src, dst
+         real(c_double), value :: dt, omega_face
+      end function
+
+      integer(c_int) function plbm_dugks_step_faces(h, iold, inew, nsteps, dt, omega_full, &
+            omega_half, omega_face) bind(c, name="plbm_dugks_step_faces")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: iold, inew, nsteps
+         real(c_double), value :: dt, omega_full, omega_half, omega_face
       end function
 
       integer(c_int) function plbm_collide(h, model, buf) bind(c, name="plbm_collide")

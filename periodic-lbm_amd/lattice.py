@@ -221,6 +221,17 @@ class Lattice:
         """dugks_stream: dst -= dt c_q . face flux balance of src."""
         check(lib().plbm_dugks_stream(self.h, src, dst, dt))
 
+    def dugks_stream_faces(self, src: int, dst: int, dt: float, omega_face: float):
+        """dugks_stream of the reference's -DDUGKS build: face collisions (rate omega_face)
+        before the flux balance."""
+        check(lib().plbm_dugks_stream_faces(self.h, src, dst, dt, omega_face))
+
+    def dugks_step_faces(self, iold: int, inew: int, nsteps: int, dt: float,
+                         omega_full: float, omega_half: float, omega_face: float):
+        """nsteps x perform_dugks_step of the -DDUGKS build (slot rules of dugks_step)."""
+        check(lib().plbm_dugks_step_faces(self.h, iold, inew, nsteps, dt, omega_full,
+                                          omega_half, omega_face))
+
     def dugks_set_steps_per_pass(self, steps: int):
         """Passes of plbm_dugks_step: 1 = collide + flux every step, 2..3 = temporally
         blocked passes, 0 = automatic (bit-identical for every value)."""
@@ -410,9 +421,28 @@ def stream_fvm_bardow(grid: LatticeGrid) -> None:
     _stream_op(grid, "fvm_bardow")
 
 
+# stream_fdm_bardow's stencil.  The reference chooses it when it is compiled
+# (src/fvm_bardow.F90:591-669: -DFDM_WLS, -DFDM_WLS_GAUSS_V1, -DFDM_WLS_GAUSS_V2,
+# -DFDM_ISO, otherwise central differences); set_fdm_bardow_stencil chooses it at run time.
+FDM_BARDOW_STENCILS = {"default": "fdm_bardow", "wls": "fdm_bardow_wls",
+                       "wls_gauss_v1": "fdm_bardow_wls_gauss_v1",
+                       "wls_gauss_v2": "fdm_bardow_wls_gauss_v2", "iso": "fdm_bardow_iso"}
+_fdm_bardow_op = "fdm_bardow"
+
+
+def set_fdm_bardow_stencil(name: str) -> None:
+    """Stencil of stream_fdm_bardow from now on: "default", "wls", "wls_gauss_v1",
+    "wls_gauss_v2" or "iso" (the reference's FDM_* build options)."""
+    global _fdm_bardow_op
+    if name not in FDM_BARDOW_STENCILS:
+        raise ValueError(f"unknown stencil {name!r}; one of {sorted(FDM_BARDOW_STENCILS)}")
+    _fdm_bardow_op = FDM_BARDOW_STENCILS[name]
+
+
 def stream_fdm_bardow(grid: LatticeGrid) -> None:
-    """stream_fdm_bardow (src/fvm_bardow.F90:511-685, default stencil) on the device."""
-    _stream_op(grid, "fdm_bardow")
+    """stream_fdm_bardow (src/fvm_bardow.F90:511-685) on the device, with the stencil of
+    set_fdm_bardow_stencil (default: central differences)."""
+    _stream_op(grid, _fdm_bardow_op)
 
 
 def stream_fdm_sofonea(grid: LatticeGrid) -> None:
@@ -452,8 +482,35 @@ def tau_over_dt(grid: LatticeGrid) -> float:
     return grid.tau / grid.dt
 
 
+# DUGKS with face collisions: the reference builds it with -DDUGKS
+# (src/periodic_dugks.F90:68,199,267,308); set_dugks_face_collisions chooses it at run time.
+_dugks_faces = False
+
+
+def set_dugks_face_collisions(enable: bool) -> None:
+    """DUGKS as the reference's -DDUGKS build from now on (True): the half collision at
+    0.75 of its rate and face collisions of rate 1/(4 tau/dt + 1) in dugks_stream."""
+    global _dugks_faces
+    _dugks_faces = bool(enable)
+
+
 def _dugks_omega_half(grid: LatticeGrid) -> float:
-    return 1.0 / (tau_over_dt(grid) + 0.5)  # src/periodic_dugks.F90:56
+    om = 1.0 / (tau_over_dt(grid) + 0.5)  # src/periodic_dugks.F90:56
+    return 0.75 * om if _dugks_faces else om  # :68-71 with -DDUGKS
+
+
+def _dugks_omega_face(grid: LatticeGrid) -> float:
+    return 1.0 / (4.0 * tau_over_dt(grid) + 1.0)  # src/periodic_dugks.F90:179
+
+
+def _dugks_steps(grid: LatticeGrid, nsteps: int) -> None:
+    if _dugks_faces:
+        grid.lattice.dugks_step_faces(grid.iold - 1, grid.inew - 1, nsteps, grid.dt,
+                                      grid.omega, _dugks_omega_half(grid),
+                                      _dugks_omega_face(grid))
+    else:
+        grid.lattice.dugks_step(grid.iold - 1, grid.inew - 1, nsteps, grid.dt, grid.omega,
+                                _dugks_omega_half(grid))
 
 
 def dugks_collide(grid: LatticeGrid) -> None:
@@ -468,7 +525,11 @@ def dugks_stream(grid: LatticeGrid) -> None:
     """dugks_stream (src/periodic_dugks.F90:168-299) on the device: f(inew) minus the
     face-flux balance of f(iold), time step grid.dt."""
     _require_device(grid)
-    grid.lattice.dugks_stream(grid.iold - 1, grid.inew - 1, grid.dt)
+    if _dugks_faces:
+        grid.lattice.dugks_stream_faces(grid.iold - 1, grid.inew - 1, grid.dt,
+                                        _dugks_omega_face(grid))
+    else:
+        grid.lattice.dugks_stream(grid.iold - 1, grid.inew - 1, grid.dt)
     _device_written(grid)
 
 
@@ -477,8 +538,7 @@ def perform_dugks_step(grid: LatticeGrid) -> None:
     swap.  With dugks_collide + dugks_stream in the slots the step runs fused."""
     if grid.collision is dugks_collide and grid.streaming is dugks_stream:
         _require_device(grid)
-        grid.lattice.dugks_step(grid.iold - 1, grid.inew - 1, 1, grid.dt, grid.omega,
-                                _dugks_omega_half(grid))
+        _dugks_steps(grid, 1)
         _device_written(grid)
     else:
         _dispatch(grid, grid.collision, _ENGINE_COLLISIONS)
@@ -492,8 +552,7 @@ def perform_dugks_steps(grid: LatticeGrid, nsteps: int) -> None:
     blocked passes (bit-identical, same slot contents)."""
     if grid.collision is dugks_collide and grid.streaming is dugks_stream:
         _require_device(grid)
-        grid.lattice.dugks_step(grid.iold - 1, grid.inew - 1, nsteps, grid.dt, grid.omega,
-                                _dugks_omega_half(grid))
+        _dugks_steps(grid, nsteps)
         _device_written(grid)
         if nsteps % 2:
             grid.iold, grid.inew = grid.inew, grid.iold
@@ -518,12 +577,17 @@ _FUSABLE_STREAMING = {lbm_stream: "exact", stream_fvm_bardow: "fvm_bardow",
                       stream_fdm_bardow: "fdm_bardow", stream_fdm_sofonea: "fdm_sofonea"}
 
 
+def _fusable_op(streaming):
+    op = _FUSABLE_STREAMING.get(streaming)
+    return _fdm_bardow_op if op == "fdm_bardow" else op
+
+
 def perform_step(grid: LatticeGrid) -> None:
     """perform_step / perform_lbm_step (src/fvm_bardow.F90:307-320,
     src/periodic_lbm.f90:15-29): a pair of engine operators (lbm_stream or a FV/FD
     streaming operator, and collide_bgk/trt/rr) runs as one fused kernel."""
     model = _ENGINE_COLLISIONS.get(grid.collision)
-    op = _FUSABLE_STREAMING.get(grid.streaming)
+    op = _fusable_op(grid.streaming)
     if model is not None and op is not None:
         _require_device(grid)
         grid.lattice.step_op(op, model, grid.iold - 1, grid.inew - 1, grid.dt)
@@ -544,7 +608,7 @@ def perform_steps(grid: LatticeGrid, nsteps: int) -> None:
     steps in FP64 storage, same slot contents); any other pair is stepped one
     perform_step at a time."""
     model = _ENGINE_COLLISIONS.get(grid.collision)
-    op = _FUSABLE_STREAMING.get(grid.streaming)
+    op = _fusable_op(grid.streaming)
     if model is not None and op is not None:
         _require_device(grid)
         iold, inew = grid.lattice.run_op(op, model, nsteps, grid.iold - 1, grid.inew - 1,

@@ -22,11 +22,19 @@ data files in this directory:
                   with the batched sweep's parameters (L = N, k = 80, delta = 0.05,
                   U0 = 0.04) at N = 32, 96, 512 and on a non-square grid with L != N:
                   full fields up to 96^2, the x- and y-profiles and sha256 at 512^2
+  ref_fdm_variants.npz  stream_fdm_bardow built with -DFDM_WLS / -DFDM_WLS_GAUSS_V1 /
+                  -DFDM_WLS_GAUSS_V2 / -DFDM_ISO (oracle/_ref/fdm_*): perform_step with BGK,
+                  5 steps, 16x16 (dt 1, 0.5) and 12x20
+  ref_dugks_faces.npz  DUGKS built with -DDUGKS (oracle/_ref/dugks): both buffers after 1
+                  and 7 steps and update_macros, the cases of ref_dugks.npz
+  ref_dugks_faces_app.json  sha256 of the fields tests/fortran/dugks_check.f90 writes
+                  through the -DDUGKS reference modules
   ref_triple.npz  perform_triple_step (src/fvm_bardow.F90:322-340): all three buffers and
                   the ids after 1 and 5 steps, TRT and BGK, 16 x 12 (nf = 3)
 
 Usage (in a container that has the reference tree and flang):
-  make -C oracle ref && python tests/golden/make_golden.py [--app | --pins-only]
+  make -C oracle ref ref-variants && python tests/golden/make_golden.py [--app | --pins-only |
+  --variants-only]
 """
 from __future__ import annotations
 
@@ -217,6 +225,80 @@ def streaming_ops(R):
     np.savez_compressed(HERE / "ref_streaming_ops.npz", **out)
 
 
+FDM_VARIANTS = {"fdm_bardow_wls": "fdm_wls", "fdm_bardow_wls_gauss_v1": "fdm_gauss_v1",
+                "fdm_bardow_wls_gauss_v2": "fdm_gauss_v2", "fdm_bardow_iso": "fdm_iso"}
+
+
+def fdm_variants():
+    """stream_fdm_bardow as the reference builds it with -DFDM_WLS, -DFDM_WLS_GAUSS_V1,
+    -DFDM_WLS_GAUSS_V2 and -DFDM_ISO (oracle/_ref/<variant>, make -C oracle ref-fdm):
+    perform_step with collide_bgk for 5 steps on a 16x16 Taylor-Green start (dt = 1 and
+    0.5) and on a seeded 12x20 field (dt = 1)."""
+    out = {}
+    rng = np.random.default_rng(77)
+    fields = {
+        "16x16": tgv_init(po.RefLib(), 16, 16, 0.05, 0.02),
+        "12x20": (np.asfortranarray(1.0 + 0.01 * rng.standard_normal((20, 12))),
+                  np.asfortranarray(0.04 * rng.standard_normal((20, 12))),
+                  np.asfortranarray(0.04 * rng.standard_normal((20, 12)))),
+    }
+    for key, (rho, ux, uy) in fields.items():
+        out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+    for op, d in FDM_VARIANTS.items():
+        RV = po.RefLib(po.HERE / "_ref" / d / "libplbm_ref.so")
+        for key, dt in (("16x16", 1.0), ("16x16", 0.5), ("12x20", 1.0)):
+            nx, ny = map(int, key.split("x"))
+            rho, ux, uy = fields[key]
+            g = RV.grid(nx, ny, "bgk")
+            g.set_properties(0.02, dt, 0.25)
+            g.set_streaming("fdm_bardow")
+            g.set_macros(rho, ux, uy)
+            g.set_pdf_to_equilibrium()
+            g.perform_step(5)
+            iold, _, _, _ = g.indices()
+            out[f"{op}_{key}_dt{dt}"] = g.get_f(iold)[:ny]
+            out[f"{op}_{key}_dt{dt}_omega"] = np.array([g.params()["omega"]])
+            g.close()
+    np.savez_compressed(HERE / "ref_fdm_variants.npz", **out)
+
+
+def dugks_faces():
+    """perform_dugks_step as the reference builds it with -DDUGKS (oracle/_ref/dugks): the
+    half-step collision at 0.75 of its rate and face collisions in the stream (rate
+    1/(4 tau/dt + 1)); both buffers after 1 and 7 steps and update_macros of f(inew), on
+    the cases of ref_dugks.npz."""
+    RD = po.RefLib(po.HERE / "_ref" / "dugks" / "libplbm_ref.so")
+    out = {}
+    cases = [(16, 16, 0.02, 1.0), (12, 20, 0.01, 0.5), (24, 8, 0.005, 2.0)]
+    for nx, ny, nu, dt in cases:
+        key = f"{nx}x{ny}_dt{dt}"
+        rng = np.random.default_rng(nx * 31 + ny + 1)
+        rho = np.asfortranarray(1.0 + 0.01 * rng.standard_normal((ny, nx)))
+        ux = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        uy = np.asfortranarray(0.05 * rng.standard_normal((ny, nx)))
+        out[f"{key}_rho"], out[f"{key}_ux"], out[f"{key}_uy"] = rho, ux, uy
+        g = RD.grid(nx, ny, "bgk")
+        g.set_properties(nu, dt, 0.25)
+        g.set_dugks()
+        g.set_macros(rho, ux, uy)
+        g.set_pdf_to_equilibrium()
+        par = g.params()
+        out[f"{key}_params"] = np.array([nu, dt, par["tau"], par["omega"]])
+        done = 0
+        for n in (1, 7):
+            g.dugks_step(n - done)
+            done = n
+            iold, inew, _, _ = g.indices()
+            out[f"{key}_{n}_iold"] = g.get_f(iold)[:ny]
+            out[f"{key}_{n}_inew"] = g.get_f(inew)[:ny]
+            out[f"{key}_{n}_ids"] = np.array([iold, inew])
+            g.update_macros()
+            r, u, v = g.macros()
+            out[f"{key}_{n}_macros"] = np.stack([r, u, v], axis=2)
+        g.close()
+    np.savez_compressed(HERE / "ref_dugks_faces.npz", **out)
+
+
 def vortex_app():
     """The unchanged app/main_vortex.f90 (128^2, stream_fvm_bardow + collide_bgk) run as
     `main_vortex 1.0`: its printout and the sha256 of the files it writes."""
@@ -274,6 +356,18 @@ def dugks_app():
                                                         indent=1))
 
 
+def dugks_faces_app():
+    """sha256 of the raw fields that tests/fortran/dugks_check.f90 writes through the
+    reference modules built with -DDUGKS (oracle/_ref/dugks/dugks_check_ref)."""
+    exe = po.HERE / "_ref" / "dugks" / "dugks_check_ref"
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, check=True)
+        files = {p.name: hashlib.sha256(p.read_bytes()).hexdigest()
+                 for p in sorted(Path(d, "out").iterdir())}
+    (HERE / "ref_dugks_faces_app.json").write_text(
+        json.dumps({"stdout": r.stdout, "files": files}, indent=1))
+
+
 def split():
     """perform_lbm_step with the -DSPLIT build's collide_bgk / collide_trt."""
     RS = po.RefLib(po.LIBREF_SPLIT)
@@ -325,6 +419,11 @@ if __name__ == "__main__":
         streaming_ops(R)
         vortex_app()
         sys.exit(0)
+    if "--variants-only" in sys.argv:
+        fdm_variants()
+        dugks_faces()
+        dugks_faces_app()
+        sys.exit(0)
     if "--pins-only" in sys.argv:
         shear(R)
         triple(R)
@@ -332,6 +431,9 @@ if __name__ == "__main__":
     if "--outputs-only" not in sys.argv:
         shear(R)
         triple(R)
+        fdm_variants()
+        dugks_faces()
+        dugks_faces_app()
         params(R)
         small(R)
         config1(R)

@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import ctypes
 import re
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -130,3 +131,33 @@ def test_sim_plugin_protocol_exported(P):
     u0 = np.zeros((4, 3, 2), order="F")
     with pytest.raises(RuntimeError):
         plugin.init((4, 3), 0.5, p0, u0)
+
+
+@pytest.mark.parametrize("define,op", [("", 2), ("-DFDM_WLS", 4), ("-DFDM_WLS_GAUSS_V1", 5),
+                                       ("-DFDM_WLS_GAUSS_V2", 6), ("-DFDM_ISO", 7)])
+def test_fortran_shim_fdm_stencil_options(tmp_path, define, op):
+    """The shim's fvm_bardow module compiles with each of the reference's stream_fdm_bardow
+    build options (src/fvm_bardow.F90:591-669) and binds stream_fdm_bardow to the engine's
+    operator of that stencil (the FDM_BARDOW_OP constant after preprocessing)."""
+    import shutil
+    import subprocess
+    flang = "/opt/rocm/lib/llvm/bin/flang"
+    if not Path(flang).exists():
+        pytest.skip("flang not available")
+    src = ROOT / "periodic-lbm_amd" / "fortran"
+    args = [flang, "-E", "-cpp"] + ([define] if define else []) + [str(src / "fvm_bardow.F90")]
+    pre = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+    pre = re.sub(r"&[ \t]*\n[ \t]*&?", "", pre)  # join free-form continuation lines
+    names = {2: "PLBM_STREAM_FDM_BARDOW", 4: "PLBM_STREAM_FDM_BARDOW_WLS",
+             5: "PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V1", 6: "PLBM_STREAM_FDM_BARDOW_WLS_GAUSS_V2",
+             7: "PLBM_STREAM_FDM_BARDOW_ISO"}
+    decl = [l for l in pre.splitlines() if "FDM_BARDOW_OP =" in l]
+    assert len(decl) == 1 and decl[0].split("=")[1].strip() == names[op]
+    text = (src / "plbm_capi.f90").read_text()
+    assert f"{names[op]} = {op}" in text
+    for mod in ("plbm_capi.f90", "precision.F90"):
+        shutil.copy(src / mod, tmp_path / mod)
+    for f in ("plbm_capi.f90", "precision.F90", "fvm_bardow.F90"):
+        cmd = [flang, "-c", "-module-dir", str(tmp_path), f"-I{tmp_path}"] + \
+              ([define] if define else []) + [str(src / f), "-o", str(tmp_path / (f + ".o"))]
+        subprocess.run(cmd, capture_output=True, text=True, check=True)

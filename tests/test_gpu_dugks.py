@@ -207,3 +207,99 @@ def test_dugks_fortran_module(gpu, tmp_path):
     for name, want in ref["files"].items():
         got = hashlib.sha256((tmp_path / "out" / name).read_bytes()).hexdigest()
         assert got == want, name
+
+
+def face_rates(nu, dt):
+    """The -DDUGKS build's rates (src/periodic_dugks.F90:56,68-71,179): full, half (0.75 of
+    the default), face."""
+    tau = 3.0 * nu
+    return dt / (tau + 0.5 * dt), 0.75 * (1.0 / (tau / dt + 0.5)), 1.0 / (4.0 * tau / dt + 1.0)
+
+
+@pytest.mark.parametrize("nx,ny,dt", [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 2.0),
+                                      (1100, 1000, 0.7)])
+def test_dugks_face_collisions_bit_exact(P, gpu, oracle, nx, ny, dt):
+    """The -DDUGKS build's DUGKS (face collisions in the stream, 0.75 half rate): the
+    operators one by one (plbm_dugks_collide + plbm_dugks_stream_faces) and the fused
+    steps (plbm_dugks_step_faces, 1 and 6 steps with the slot rules) against the oracle,
+    both slots, bit for bit."""
+    print(f"case dugks faces {nx}x{ny} dt={dt}", flush=True)
+    rho, ux, uy = fields(nx, ny, nx * 5 + ny)
+    om_full, om_half, om_face = face_rates(0.01, dt)
+    lat = P.Lattice(nx, ny, 2, "f64")
+    lat.equilibrium_init(0, rho, ux, uy)
+    a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+    for s in range(4):
+        src, dst = s % 2, 1 - s % 2
+        lat.dugks_collide(src, dst, om_full, om_half)
+        lat.dugks_stream_faces(src, dst, dt, om_face)
+        oracle.dugks_collide(nx, ny, a, b, om_full, om_half)
+        oracle.dugks_stream_face(nx, ny, a, b, dt, om_face)
+        a, b = b, a
+    np.testing.assert_array_equal(lat.download_f(0)[:ny], a[:ny])
+    np.testing.assert_array_equal(lat.download_f(1)[:ny], b[:ny])
+    for nsteps in (1, 6):
+        f = P.Lattice(nx, ny, 2, "f64")
+        f.equilibrium_init(0, rho, ux, uy)
+        f.dugks_step_faces(0, 1, nsteps, dt, om_full, om_half, om_face)
+        a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+        oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+        for _ in range(nsteps):
+            oracle.dugks_collide(nx, ny, a, b, om_full, om_half)
+            oracle.dugks_stream_face(nx, ny, a, b, dt, om_face)
+            a, b = b, a
+        # the reference swaps its ids each step: slot 0 holds buffer a after an even count
+        latest, other = (0, 1) if nsteps % 2 == 0 else (1, 0)
+        np.testing.assert_array_equal(f.download_f(latest)[:ny], a[:ny], err_msg=str(nsteps))
+        np.testing.assert_array_equal(f.download_f(other)[:ny], b[:ny], err_msg=str(nsteps))
+        f.close()
+
+
+def test_dugks_face_collisions_python_api(P, gpu, oracle):
+    """The reference-named Python API with set_dugks_face_collisions(True): the fused
+    perform_dugks_step and perform_dugks_steps against the oracle (the rates from tau and
+    dt as the -DDUGKS build computes them)."""
+    nx, ny, nu, dt = 96, 80, 0.01, 0.8
+    rho, ux, uy = fields(nx, ny, 17)
+    om_full, om_half, om_face = face_rates(nu, dt)
+    a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+    for _ in range(5):
+        oracle.dugks_collide(nx, ny, a, b, om_full, om_half)
+        oracle.dugks_stream_face(nx, ny, a, b, dt, om_face)
+        a, b = b, a
+    P.set_dugks_face_collisions(True)
+    try:
+        for mode in ("steps", "one-by-one"):
+            g = P.alloc_grid(nx, ny, 2, log=False)
+            g.mf[:, :, 0], g.mf[:, :, 1], g.mf[:, :, 2] = rho, ux, uy
+            P.set_properties(g, nu, dt, 0.25)
+            g.collision, g.streaming = P.dugks_collide, P.dugks_stream
+            P.set_pdf_to_equilibrium(g)
+            if mode == "steps":
+                P.perform_dugks_steps(g, 5)
+            else:
+                for _ in range(5):
+                    P.perform_dugks_step(g)
+            P.sync_host(g)
+            np.testing.assert_array_equal(g.f[:ny, :, :, g.iold - 1], a[:ny], err_msg=mode)
+            np.testing.assert_array_equal(g.f[:ny, :, :, g.inew - 1], b[:ny], err_msg=mode)
+            P.dealloc_grid(g)
+    finally:
+        P.set_dugks_face_collisions(False)
+
+
+def test_dugks_face_collisions_fortran_module(gpu, tmp_path):
+    """tests/fortran/dugks_check.f90 linked against the shim compiled with -DDUGKS writes
+    the same bytes as through the reference's modules compiled with -DDUGKS."""
+    exe = ROOT / "periodic-lbm_amd" / "_build" / "tests" / "dugks_check_faces"
+    if not exe.exists():
+        pytest.skip("Fortran test programs not built")
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = json.loads((GOLDEN / "ref_dugks_faces_app.json").read_text())
+    assert r.stdout.split() == ref["stdout"].split()
+    for name, want in ref["files"].items():
+        got = hashlib.sha256((tmp_path / "out" / name).read_bytes()).hexdigest()
+        assert got == want, name

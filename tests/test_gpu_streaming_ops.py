@@ -15,6 +15,9 @@ import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 OPS = ["fvm_bardow", "fdm_bardow", "fdm_sofonea"]
+# stream_fdm_bardow's compile-time stencils (src/fvm_bardow.F90:591-669)
+FDM_VARIANTS = ["fdm_bardow_wls", "fdm_bardow_wls_gauss_v1", "fdm_bardow_wls_gauss_v2",
+                "fdm_bardow_iso"]
 
 
 def fields(nx, ny, seed):
@@ -29,7 +32,7 @@ def fields(nx, ny, seed):
 SIZES = [(40, 40, 1.0), (33, 70, 0.45), (130, 5, 0.8), (1100, 1000, 0.7), (3300, 2500, 1.0)]
 
 
-@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("op", OPS + FDM_VARIANTS)
 @pytest.mark.parametrize("nx,ny,dt", SIZES)
 def test_streaming_operator_bit_exact(P, gpu, oracle, op, nx, ny, dt):
     print(f"case {op} {nx}x{ny} dt={dt}", flush=True)
@@ -51,7 +54,7 @@ def test_streaming_operator_bit_exact(P, gpu, oracle, op, nx, ny, dt):
 
 
 @pytest.mark.parametrize("model", ["bgk", "trt", "rr"])
-@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("op", OPS + FDM_VARIANTS)
 @pytest.mark.parametrize("nx,ny,dt", [SIZES[0], SIZES[2], SIZES[3], SIZES[4]])
 def test_fused_operator_step_bit_exact(P, gpu, oracle, op, model, nx, ny, dt):
     """plbm_step_op (stream operator + collision in one pass) against the oracle's
@@ -225,3 +228,44 @@ def test_operator_passes_full_size(P, gpu, oracle):
         oracle.collide("bgk", n, n, b, omega)
         a, b = b, a
     np.testing.assert_array_equal(lat.download_f(i)[:n], a[:n])
+
+
+@pytest.mark.parametrize("op", FDM_VARIANTS)
+def test_fdm_stencil_variants_in_passes(P, gpu, op):
+    """The -DFDM_* stencils through plbm_run_op in passes of 2 and 3 steps (with the
+    lagged slot from the last pass) against single fused steps, both slots, FP64; and the
+    reference-named Python API with the stencil chosen by set_fdm_bardow_stencil."""
+    nx, ny, nsteps, dt = 200, 130, 7, 0.8
+    print(f"case {op} passes {nx}x{ny}", flush=True)
+    rho, ux, uy = fields(nx, ny, 41)
+    out = []
+    for pp in (1, 2, 3):
+        lat = P.Lattice(nx, ny, 2, "f64")
+        lat.set_params(1.0 / (3 * 0.01 / dt + 0.5), 0.25)
+        lat.set_op_steps_per_pass(pp)
+        lat.equilibrium_init(0, rho, ux, uy)
+        i, n = lat.run_op(op, "trt", nsteps, 0, 1, dt)
+        out.append((i, n, lat.download_f(i), lat.download_f(n)))
+        lat.close()
+    for i, n, a, b in out[1:]:
+        assert (i, n) == out[0][:2]
+        np.testing.assert_array_equal(a, out[0][2])
+        np.testing.assert_array_equal(b, out[0][3])
+    name = [k for k, v in P.FDM_BARDOW_STENCILS.items() if v == op][0]
+    g = P.alloc_grid(nx, ny, 2, log=False)
+    g.mf[:, :, 0], g.mf[:, :, 1], g.mf[:, :, 2] = rho, ux, uy
+    P.set_properties(g, 0.01, dt, 0.25)
+    g.streaming, g.collision = P.stream_fdm_bardow, P.collide_trt
+    P.set_pdf_to_equilibrium(g)
+    P.set_fdm_bardow_stencil(name)
+    try:
+        P.perform_steps(g, nsteps)
+    finally:
+        P.set_fdm_bardow_stencil("default")
+    # the grid starts from the reference's ids (iold = 2, inew = 1): after an odd count
+    # its latest state is in buffer 1 (0-based 0)
+    assert (g.iold, g.inew) == (1, 2)
+    P.sync_host(g)
+    np.testing.assert_array_equal(g.f[:, :, :, g.iold - 1], out[0][2])
+    np.testing.assert_array_equal(g.f[:, :, :, g.inew - 1], out[0][3])
+    P.dealloc_grid(g)

@@ -136,6 +136,42 @@ def test_dugks_bitwise(oracle, nx, ny, dt):
         np.testing.assert_array_equal(np.stack([r, u, v], axis=2), G[f"{key}_{n}_macros"])
 
 
+@pytest.mark.parametrize("nx,ny,dt", DUGKS_CASES)
+def test_dugks_face_collision_build_bitwise(oracle, nx, ny, dt):
+    """perform_dugks_step of the -DDUGKS build (src/periodic_dugks.F90:68,199,267,308): the
+    half-step collision at 0.75 of 1/(tau/dt + 1/2) and the stream with face collisions of
+    rate 1/(4 tau/dt + 1); both buffers after 1 and 7 steps and update_macros of f(inew),
+    against that build's own run (ref_dugks_faces.npz)."""
+    G = np.load(GOLDEN / "ref_dugks_faces.npz")
+    key = f"{nx}x{ny}_dt{dt}"
+    rho, ux, uy = (np.asfortranarray(G[f"{key}_{c}"]) for c in ("rho", "ux", "uy"))
+    nu, dt_, tau, omega = G[f"{key}_params"]
+    par = oracle.set_properties(nu, dt, 0.25)
+    assert (par["tau"], par["omega"]) == (tau, omega)
+    om_half = 0.75 * (1.0 / (tau / dt + 0.5))
+    om_face = 1.0 / (4.0 * (tau / dt) + 1.0)
+    bufs = [po.new_f(nx, ny), po.new_f(nx, ny)]
+    inew, iold = (int(i) for i in G[f"{key}_1_ids"])
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, bufs[iold - 1])
+    done = 0
+    for n in (1, 7):
+        for _ in range(n - done):
+            oracle.dugks_collide(nx, ny, bufs[iold - 1], bufs[inew - 1], omega, om_half)
+            oracle.dugks_stream_face(nx, ny, bufs[iold - 1], bufs[inew - 1], dt, om_face)
+            iold, inew = inew, iold
+        done = n
+        assert [iold, inew] == list(G[f"{key}_{n}_ids"])
+        np.testing.assert_array_equal(bufs[iold - 1][:ny], G[f"{key}_{n}_iold"])
+        np.testing.assert_array_equal(bufs[inew - 1][:ny], G[f"{key}_{n}_inew"])
+        r, u, v = oracle.update_macros(nx, ny, bufs[inew - 1])
+        np.testing.assert_array_equal(np.stack([r, u, v], axis=2), G[f"{key}_{n}_macros"])
+    # the default build's step from the same start gives another result
+    a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+    a, b = oracle.dugks_run(nx, ny, a, b, 7, dt, omega, 1.0 / (tau / dt + 0.5))
+    assert not np.array_equal(a[:ny], G[f"{key}_7_iold"])
+
+
 @pytest.mark.parametrize("nx,ny", [(16, 16), (12, 20)])
 def test_split_build_bitwise(oracle, nx, ny):
     """collide_bgk of the -DSPLIT build (bgk_kernel_cache, src/collision_bgk.F90:84-176)
@@ -228,3 +264,27 @@ def test_config1_l2_norm_against_reference(oracle, model):
     _, u, v = oracle.update_macros(n, n, prev)
     _, ua, va = oracle.tgv(n, n, k, k, u0, nu, float(steps))
     assert oracle.l2_norm(u, v, ua, va) == pytest.approx(C["models"][model]["l2_lag"], rel=1e-13)
+
+
+FDM_VARIANTS = ["fdm_bardow_wls", "fdm_bardow_wls_gauss_v1", "fdm_bardow_wls_gauss_v2",
+                "fdm_bardow_iso"]
+
+
+@pytest.mark.parametrize("key,dt", [("16x16", 1.0), ("16x16", 0.5), ("12x20", 1.0)])
+@pytest.mark.parametrize("op", FDM_VARIANTS)
+def test_fdm_stencil_variants_bitwise(oracle, op, key, dt):
+    """stream_fdm_bardow's compile-time stencils (src/fvm_bardow.F90:591-669; -DFDM_WLS,
+    -DFDM_WLS_GAUSS_V1/_V2, -DFDM_ISO) with collide_bgk, 5 steps, against the reference
+    built with each option (ref_fdm_variants.npz)."""
+    G = np.load(GOLDEN / "ref_fdm_variants.npz")
+    nx, ny = map(int, key.split("x"))
+    rho, ux, uy = (np.asfortranarray(G[f"{key}_{k}"]) for k in ("rho", "ux", "uy"))
+    omega = float(G[f"{op}_{key}_dt{dt}_omega"][0])
+    assert omega == oracle.set_properties(0.02, dt, 0.25)["omega"]
+    a, b = po.new_f(nx, ny), po.new_f(nx, ny)
+    oracle.set_pdf_to_equilibrium(nx, ny, rho, ux, uy, a)
+    for _ in range(5):
+        oracle.stream_op(op, nx, ny, a, b, dt)
+        oracle.collide("bgk", nx, ny, b, omega)
+        a, b = b, a
+    np.testing.assert_array_equal(a[:ny], G[f"{op}_{key}_dt{dt}"])
