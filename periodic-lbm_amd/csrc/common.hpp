@@ -169,6 +169,7 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
 // lattice's third buffer (nullptr when it cannot be allocated), and the final assignment
 // of device storage to slots.
 std::vector<int> pass_plan(int nsteps, int ns);
+std::vector<int> lag_pass_plan(int nsteps, int ns);
 void *pass_aux(plbm_lattice *h);
 void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void *lag,
                 void *spare, int *iold, int *inew);

@@ -132,17 +132,28 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
     });
 }
 
-// Depths of the passes of a run of nsteps >= 2 steps with passes of at most ns >= 2 steps:
-// the fewest passes, their depths differing by at most one, the deepest last (so the last
-// pass, which also stores the state one step before the latest, has at least 2 steps).
-// A pass of depth d costs about one HBM sweep for every d up to 6 at 8192^2 (TRT FP64,
-// us per launch for d = 2..8: 1969 2004 1995 1952 1992 2249 2447), so splitting 20 steps
-// as 6 + 7 + 7 costs less than 8 + 8 + 4 (6490 against 6889 us) and much less than
-// 8 + 8 + 3 followed by a single-step sweep (8587 us).
+// Depths of the passes of nsteps >= 1 steps with passes of at most ns >= 1 steps: the
+// fewest passes, their depths differing by at most one, the deepest last.  A pass of
+// depth d costs about one HBM sweep for every d up to 6 at 8192^2 (TRT FP64, us per
+// launch for d = 2..8: 1969 2004 1995 1952 1992 2249 2447), so 12 steps as 6 + 6 cost
+// less than 8 + 4 (3984 against 4442 us).
 std::vector<int> pass_plan(int nsteps, int ns) {
     const int passes = (nsteps + ns - 1) / ns;
     std::vector<int> d((size_t)passes, nsteps / passes);
     for (int p = 0; p < nsteps % passes; ++p) d[(size_t)(passes - 1 - p)] += 1;
+    return d;
+}
+
+// Depths of a run of nsteps >= 2 steps whose last pass also stores the state one step
+// before the latest: that pass writes the whole state twice and is bound by HBM writes
+// whatever its depth (8192^2 TRT FP64, ms per launch for d = 2..8 with / without the
+// second store: 3.02 / 1.97, 3.08 / 2.02, 3.03 / 2.01, 2.93 / 1.98, 2.92 / 2.06, 3.00 /
+// 2.40, 2.99 / 2.62), so it takes as many steps as a pass holds and the rest is split as
+// pass_plan does: 20 steps at depth 8 as 6 + 6 + 8 (7.1 ms) instead of 6 + 7 + 7 (7.5 ms).
+std::vector<int> lag_pass_plan(int nsteps, int ns) {
+    const int last = std::min(ns, nsteps);
+    std::vector<int> d = nsteps > last ? pass_plan(nsteps - last, ns) : std::vector<int>();
+    d.push_back(last);
     return d;
 }
 
@@ -178,7 +189,8 @@ void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void 
 }
 
 // nsteps steps of `model` with the reference's bookkeeping (each step reads *iold, writes
-// *inew, swaps them).  With ns > 1 the steps run in passes of up to ns steps (pass_plan);
+// *inew, swaps them).  With ns > 1 the steps run in passes of up to ns steps
+// (lag_pass_plan);
 // the last pass also stores the state one step before the latest into the third buffer,
 // which becomes the other slot's storage, as update_macros expects of it (the reference's
 // inew after perform_lbm_step).  Without a third buffer the last step is a single-step
@@ -226,7 +238,7 @@ int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *in
         finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
         return PLBM_OK;
     }
-    const std::vector<int> plan = pass_plan(nsteps, ns);
+    const std::vector<int> plan = lag_pass_plan(nsteps, ns);
     for (size_t p = 0; p + 1 < plan.size(); ++p) {
         PLBM_TRY(launch(plan[p], nullptr));
         std::swap(latest, other);

@@ -125,14 +125,14 @@ extern "C" int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *
         }
         return PLBM_OK;
     }
-    // passes of up to ns steps, evenly split, the deepest last; the last pass also stores
-    // the state one step before the latest into the third buffer, which becomes the other
-    // slot's storage (as after perform_step).  Without a third buffer the last step is a
-    // single fused step instead.
+    // passes of up to ns steps; the last, as deep as a pass goes, also stores the state one
+    // step before the latest into the third buffer, which becomes the other slot's storage
+    // (as after perform_step).  Without a third buffer the last step is a single fused step
+    // instead.
     const int a0 = *iold, b0 = *inew;
     void *latest = h->f[a0], *other = h->f[b0];
     void *aux = pass_aux(h);
-    const std::vector<int> plan = pass_plan(aux ? nsteps : nsteps - 1, ns);
+    const std::vector<int> plan = aux ? lag_pass_plan(nsteps, ns) : pass_plan(nsteps - 1, ns);
     for (size_t p = 0; p + 1 < plan.size(); ++p) {
         if (plan[p] == 1) PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
         else PLBM_TRY(launch_fd_multi(h, op, model, plan[p], latest, other, dt));

@@ -3,6 +3,8 @@
   python periodic-lbm_amd/rocprof_summary.py stats  <kernel_stats.csv> <out.md> --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py trace  <kernel_trace.csv> <out.md>
          --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N --cmd "<cmd>"
+  python periodic-lbm_amd/rocprof_summary.py window <kernel_trace.csv> <out.md>
+         --sequence <name substring> ... [--bytes B ...] --skip K --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> <out.json> --kernel <substring> --cells N
          --bytes-per-lu B --cmd "<cmd>"
@@ -42,14 +44,18 @@ def trace(args):
     """Per-kernel durations from kernel_trace.csv, restricted to the last --last dispatches
     of each selected kernel (the timed region of bench.py, after warm-up and tuning)."""
     rows = _rows(args.csv)
-    lines = [f"# rocprofv3 kernel trace: timed-region dispatches", "", f"Command: `{args.cmd}`", "",
-             f"Last {args.last} dispatches of each kernel below (the timed steps).", "",
+    lines = [f"# rocprofv3 kernel trace: dispatch durations", "", f"Command: `{args.cmd}`", "",
+             f"Last {args.last} dispatches of each kernel below"
+             + (f" with grid size {', '.join(map(str, args.grid_size))}" if args.grid_size else "")
+             + ".", "",
              "| kernel | dispatches | avg us | min us | max us | bytes/launch | GB/s at avg |",
              "|---|---:|---:|---:|---:|---:|---:|"]
     for spec in args.kernel:
         sub, nbytes = spec.rsplit("=", 1)
         d = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
-              r["Kernel_Name"]) for r in rows if sub in r["Kernel_Name"]]
+              r["Kernel_Name"]) for r in rows if sub in r["Kernel_Name"]
+             and (not args.grid_size or int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+                  in args.grid_size)]
         d.sort()
         sel = [x[1] for x in d[-args.last:]]
         if not sel:
@@ -61,20 +67,58 @@ def trace(args):
     Path(args.out).write_text("\n".join(lines) + "\n")
 
 
-def _per_kernel(rows, counter):
+def window(args):
+    """The dispatches of one stretch of the run, located as the first place in the trace
+    where consecutive dispatches match the given kernel-name substrings in order (for
+    bench.py's timed region: its warm-up pass, then the timed passes)."""
+    rows = _rows(args.csv)
+    d = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                r["Kernel_Name"]) for r in rows)
+    seq = args.sequence
+    start = next((i for i in range(len(d) - len(seq) + 1)
+                  if all(seq[j] in d[i + j][2] for j in range(len(seq)))), None)
+    if start is None:
+        raise SystemExit(f"sequence {seq} not found")
+    lines = [f"# rocprofv3 kernel trace: {args.title}", "", f"Command: `{args.cmd}`", "",
+             "Consecutive dispatches located by their kernel names (the first "
+             f"{args.skip} of them precede the timed region).", "",
+             "| # | kernel | us | algorithmic bytes | GB/s |", "|---:|---|---:|---:|---:|"]
+    total = 0.0
+    for j in range(len(seq)):
+        t, dur, name = d[start + j]
+        nb = args.bytes[j] if j < len(args.bytes) else 0.0
+        timed = j >= args.skip
+        total += dur if timed else 0.0
+        lines.append(f"| {j}{'' if timed else ' (before)'} | `{name[:60]}` | {dur / 1e3:.1f} | "
+                     f"{nb:.4g} | {nb / dur if nb else 0:.1f} |")
+    lines += ["", f"Timed dispatches: {len(seq) - args.skip}, {total / 1e3:.1f} us in total."]
+    Path(args.out).write_text("\n".join(lines) + "\n")
+
+
+def _per_kernel(rows, counter, grid_size=None):
+    """Counter values per kernel name; with grid_size, only the dispatches of the kernel
+    itself with that grid size (the same kernel also runs on other lattices)."""
     agg = {}
     for r in rows:
         if r["Counter_Name"] != counter:
             continue
-        agg.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        agg.setdefault(r["Kernel_Name"], []).append(
+            (float(r["Counter_Value"]), int(r.get("Grid_Size") or 0)))
     return agg
 
 
-def _pick(agg, sub):
+def _values(agg, name, grid_size=None):
+    return [v for v, g in agg[name] if grid_size is None or g == grid_size]
+
+
+def _pick(agg, sub, grid_size=None):
     keys = [k for k in agg if sub in k]
     if not keys:
         raise SystemExit(f"no kernel matching {sub!r} in {list(agg)}")
-    return keys[0], statistics.median(agg[keys[0]])
+    vals = _values(agg, keys[0], grid_size)
+    if not vals:
+        raise SystemExit(f"no dispatch of {keys[0]!r} with grid size {grid_size}")
+    return keys[0], statistics.median(vals)
 
 
 def traffic(args):
@@ -85,13 +129,14 @@ def traffic(args):
     copy_bytes = args.copy_bytes
     f_scale = copy_bytes / (copy_fetch * 1024.0)
     w_scale = copy_bytes / (copy_write * 1024.0)
-    kname, kfetch = _pick(fetch, args.kernel)
-    _, kwrite = _pick(write, args.kernel)
+    kname, kfetch = _pick(fetch, args.kernel, args.grid_size)
+    _, kwrite = _pick(write, args.kernel, args.grid_size)
     read_b = kfetch * 1024.0 * f_scale
     write_b = kwrite * 1024.0 * w_scale
     algo = args.cells * args.bytes_per_lu
     rec = {
-        "kernel": kname, "command": args.cmd,
+        "kernel": kname, "command": args.cmd, "grid_size": args.grid_size,
+        "dispatches": len(_values(fetch, kname, args.grid_size)),
         "fetch_kib_per_dispatch": kfetch, "write_kib_per_dispatch": kwrite,
         "calibration": {"kernel": copy_name, "bytes_read_and_written": copy_bytes,
                         "fetch_kib": copy_fetch, "write_kib": copy_write,
@@ -118,6 +163,8 @@ def main():
     b.add_argument("--cells", type=float, required=True)
     b.add_argument("--bytes-per-lu", type=float, required=True)
     b.add_argument("--copy-bytes", type=float, default=float(4 << 30))
+    b.add_argument("--grid-size", type=int, default=None,
+                   help="only dispatches of the kernel with this grid size (threads)")
     b.add_argument("--cmd", required=True)
     c = sub.add_parser("trace")
     c.add_argument("csv")
@@ -125,9 +172,21 @@ def main():
     c.add_argument("--kernel", action="append", required=True,
                    help="<name substring>=<algorithmic bytes per launch>, repeatable")
     c.add_argument("--last", type=int, default=50)
+    c.add_argument("--grid-size", type=int, action="append", default=[],
+                   help="only dispatches with one of these grid sizes (threads)")
     c.add_argument("--cmd", required=True)
+    w = sub.add_parser("window")
+    w.add_argument("csv")
+    w.add_argument("out")
+    w.add_argument("--sequence", nargs="+", required=True,
+                   help="kernel-name substrings of consecutive dispatches")
+    w.add_argument("--bytes", nargs="*", type=float, default=[],
+                   help="algorithmic bytes per launch of each dispatch in the sequence")
+    w.add_argument("--skip", type=int, default=0, help="leading dispatches before the window")
+    w.add_argument("--title", default="timed region")
+    w.add_argument("--cmd", required=True)
     args = ap.parse_args()
-    {"stats": stats, "traffic": traffic, "trace": trace}[args.mode](args)
+    {"stats": stats, "traffic": traffic, "trace": trace, "window": window}[args.mode](args)
 
 
 if __name__ == "__main__":

@@ -32,7 +32,7 @@ def headline_fields(P, n, u0=0.05, nu=0.01, seed=7):
 @pytest.mark.parametrize("model,steps", [("trt", 16), ("rr", 16), ("trt", 17)])
 def test_headline_passes_bit_exact_8192(P, gpu, oracle, model, steps):
     """8192^2 FP64 through plbm_run at the automatic depth (8): 16 steps are two full
-    8-step passes (the second also writes the lagged slot), 17 steps split 5 + 6 + 6.
+    8-step passes (the second also writes the lagged slot), 17 steps run as 4 + 5 + 8.
     Latest and lagged slot against the C oracle, bit for bit."""
     n = 8192
     print(f"case headline {model} {n}^2 {steps} steps", flush=True)
