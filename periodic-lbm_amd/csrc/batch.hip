@@ -608,6 +608,7 @@ int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld) {
     PLBM_TRY(ensure_btmp(b, n * sizeof(double)));
     const size_t eb = storage_bytes(b->storage);
     const void *src = static_cast<const char *>(b->arena) + (k.off + (b->parity ? k.elems : 0)) * eb;
+    std::vector<double> stage;
     for (int q = 0; q < 9; ++q) {
         PLBM_TRY(dispatch_storage(b->storage, [&](auto S) {
             hipLaunchKernelGGL(k_batch_to_f64<decltype(S)::value>, dim3((k.ny + 255) / 256, k.nx),
@@ -616,10 +617,8 @@ int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld) {
             PLBM_HIP(hipGetLastError());
             return PLBM_OK;
         }));
-        PLBM_HIP(hipMemcpy2DAsync(f + (size_t)q * ld * k.nx, (size_t)ld * 8, b->d_tmp,
-                                  (size_t)k.ny * 8, (size_t)k.ny * 8, k.nx, hipMemcpyDeviceToHost,
-                                  b->stream));
-        PLBM_HIP(hipStreamSynchronize(b->stream));
+        PLBM_TRY(plane_to_host(f + (size_t)q * ld * k.nx, ld, b->d_tmp, k.ny, k.nx, b->stream,
+                               stage));
     }
     return PLBM_OK;
 }
@@ -632,10 +631,10 @@ int plbm_batch_upload_f(plbm_batch *b, int c, const double *f, int ld) {
     PLBM_TRY(ensure_btmp(b, n * sizeof(double)));
     const size_t eb = storage_bytes(b->storage);
     void *dst = static_cast<char *>(b->arena) + (k.off + (b->parity ? k.elems : 0)) * eb;
+    std::vector<double> stage;
     for (int q = 0; q < 9; ++q) {
-        PLBM_HIP(hipMemcpy2DAsync(b->d_tmp, (size_t)k.ny * 8, f + (size_t)q * ld * k.nx,
-                                  (size_t)ld * 8, (size_t)k.ny * 8, k.nx, hipMemcpyHostToDevice,
-                                  b->stream));
+        PLBM_TRY(plane_from_host(b->d_tmp, f + (size_t)q * ld * k.nx, ld, k.ny, k.nx, b->stream,
+                                 stage));
         PLBM_TRY(dispatch_storage(b->storage, [&](auto S) {
             hipLaunchKernelGGL(k_batch_from_f64<decltype(S)::value>, dim3((k.ny + 255) / 256, k.nx),
                                dim3(256), 0, b->stream, b->d_tmp, dst, q, k.nx, k.ny, k.ldd, wq(q),

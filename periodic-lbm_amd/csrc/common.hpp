@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -155,6 +156,39 @@ constexpr int kModels = 4;
 constexpr int kMaxStepsPerLaunch = 8;  // temporal-blocking depth of plbm_run
 int max_steps_per_pass(int storage);  // PLBM_BGK, PLBM_TRT, PLBM_RR, PLBM_BGK_SPLIT
 constexpr int kReduceBlocks = 1024;
+
+// Host plane of nx columns with pitch ld (ld >= ny, the host arrays of the upload /
+// download entry points) to / from a compact device plane (pitch ny), synchronous on
+// return.  Only 1-D copies: a pitched host plane goes through `stage` (host memory).  A
+// pitched 2-D asynchronous copy between device and pageable host memory delivered no data
+// when the process ran under rocprofv3's kernel tracing (ROCm 7.2; the host array kept
+// its prior contents), while 1-D copies did.
+inline int plane_to_host(double *host, size_t ld, const double *dev, size_t ny, size_t nx,
+                         hipStream_t s, std::vector<double> &stage) {
+    const bool direct = ld == ny;
+    if (!direct) stage.resize(ny * nx);
+    PLBM_HIP(hipMemcpyAsync(direct ? host : stage.data(), dev, ny * nx * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    PLBM_HIP(hipStreamSynchronize(s));
+    if (!direct)
+        for (size_t x = 0; x < nx; ++x)
+            std::copy(stage.data() + x * ny, stage.data() + (x + 1) * ny, host + x * ld);
+    return PLBM_OK;
+}
+inline int plane_from_host(double *dev, const double *host, size_t ld, size_t ny, size_t nx,
+                           hipStream_t s, std::vector<double> &stage) {
+    const bool direct = ld == ny;
+    if (!direct) {
+        stage.resize(ny * nx);
+        for (size_t x = 0; x < nx; ++x)
+            std::copy(host + x * ld, host + x * ld + ny, stage.data() + x * ny);
+    }
+    PLBM_HIP(hipMemcpyAsync(dev, direct ? host : stage.data(), ny * nx * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    // the source (caller's array or stage) must outlive the copy
+    PLBM_HIP(hipStreamSynchronize(s));
+    return PLBM_OK;
+}
 
 }  // namespace plbm
 

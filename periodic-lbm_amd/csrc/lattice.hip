@@ -629,10 +629,11 @@ int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) {
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
     PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
+    std::vector<double> stage;
     for (int q = 0; q < 9; ++q) {
         // host plane (pitch ld) -> compact staging plane (pitch ny)
-        PLBM_HIP(hipMemcpy2DAsync(h->d_tmp, (size_t)h->ny * 8, f + q * plane_host, (size_t)ld * 8,
-                                  (size_t)h->ny * 8, h->nx, hipMemcpyHostToDevice, h->stream));
+        PLBM_TRY(plane_from_host(h->d_tmp, f + q * plane_host, ld, h->ny, h->nx, h->stream,
+                                 stage));
         const double w = wq(q);
         PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
             hipLaunchKernelGGL(k_plane_from_f64<decltype(S)::value>, cell_grid(h), dim3(256), 0,
@@ -651,6 +652,7 @@ int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
     PLBM_TRY(ensure_tmp(h, (size_t)h->nx * h->ny * sizeof(double)));
+    std::vector<double> stage;
     for (int q = 0; q < 9; ++q) {
         const double w = wq(q);
         PLBM_TRY(dispatch_storage(h->storage, [&](auto S) {
@@ -661,9 +663,7 @@ int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
             return PLBM_OK;
         }));
         // compact staging plane -> host plane (pitch ld); padding rows ny..ld-1 untouched
-        PLBM_HIP(hipMemcpy2DAsync(f + q * plane_host, (size_t)ld * 8, h->d_tmp, (size_t)h->ny * 8,
-                                  (size_t)h->ny * 8, h->nx, hipMemcpyDeviceToHost, h->stream));
-        PLBM_HIP(hipStreamSynchronize(h->stream));
+        PLBM_TRY(plane_to_host(f + q * plane_host, ld, h->d_tmp, h->ny, h->nx, h->stream, stage));
     }
     return PLBM_OK;
 }

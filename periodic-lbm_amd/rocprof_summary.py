@@ -5,6 +5,9 @@
          --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py window <kernel_trace.csv> <out.md>
          --sequence <name substring> ... [--bytes B ...] --skip K --cmd "<cmd>"
+  python periodic-lbm_amd/rocprof_summary.py sq <counter_collection.csv> <kernel_trace.csv>
+         <out.md> --kernel <substring> [--kernel ...] --grid-size N --cmd "<pmc cmd>"
+         --trace-cmd "<trace cmd>"
   python periodic-lbm_amd/rocprof_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> <out.json> --kernel <substring> --cells N
          --bytes-per-lu B --cmd "<cmd>"
@@ -148,6 +151,53 @@ def traffic(args):
     Path(args.out).write_text(json.dumps(rec, indent=1) + "\n")
 
 
+SQ_COUNTERS = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
+               "SQ_INSTS_VMEM_WR", "SQ_WAVES"]
+
+
+def sq(args):
+    """Instruction counts per dispatch (median over the kernel's dispatches with the given
+    grid sizes) beside the launch time of the same dispatches in a kernel trace, and the
+    share of the VALU issue capacity they take: SQ_INSTS_VALU x 4 cycles (a wave64 FP64 or
+    unpacked FP32 instruction on a 16-lane SIMD) / (SIMDs x clock x launch time)."""
+    crow = _rows(args.counters)
+    trows = _rows(args.trace)
+    grids = set(args.grid_size)
+
+    def grid_ok(r, key):
+        return not grids or int(r.get(key) or 0) in grids
+
+    lines = [f"# rocprofv3 instruction counters (SQ), {args.title}", "",
+             f"Counters: `{args.cmd}`", "", f"Launch times: `{args.trace_cmd}`", "",
+             "Medians per dispatch over the kernel's dispatches"
+             + (f" with grid size {', '.join(map(str, args.grid_size))}" if grids else "")
+             + " (millions of wavefront instructions; waves in units).  VALU busy = "
+             f"SQ_INSTS_VALU x 4 / ({args.simds} SIMDs x {args.clock_ghz} GHz x launch time).",
+             "", "| kernel | VALU | SALU | LDS | VMEM rd | VMEM wr | waves | launch ms | VALU busy |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for sub in args.kernel:
+        vals = {}
+        name = None
+        for r in crow:
+            if sub in r["Kernel_Name"] and r["Counter_Name"] in SQ_COUNTERS and grid_ok(r, "Grid_Size"):
+                vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                name = r["Kernel_Name"]
+        if name is None:
+            raise SystemExit(f"no counter rows for {sub!r}")
+        med = {k: statistics.median(v) for k, v in vals.items()}
+        durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trows
+                if sub in r["Kernel_Name"] and grid_ok(r, "Grid_Size_X")]
+        if not durs:
+            raise SystemExit(f"no trace dispatch of {sub!r}")
+        t = statistics.median(durs) * 1e-9
+        busy = med.get("SQ_INSTS_VALU", 0.0) * 4 / (args.simds * args.clock_ghz * 1e9 * t)
+        m = lambda k: f"{med.get(k, float('nan')) / 1e6:.2f}"
+        lines.append(f"| `{name[:46]}` | {m('SQ_INSTS_VALU')} | {m('SQ_INSTS_SALU')} | "
+                     f"{m('SQ_INSTS_LDS')} | {m('SQ_INSTS_VMEM_RD')} | {m('SQ_INSTS_VMEM_WR')} | "
+                     f"{med.get('SQ_WAVES', 0):.0f} | {t * 1e3:.4f} | {busy:.2f} |")
+    Path(args.out).write_text("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -185,8 +235,21 @@ def main():
     w.add_argument("--skip", type=int, default=0, help="leading dispatches before the window")
     w.add_argument("--title", default="timed region")
     w.add_argument("--cmd", required=True)
+    q = sub.add_parser("sq")
+    q.add_argument("counters", help="counter_collection.csv of a --pmc run of SQ_ counters")
+    q.add_argument("trace", help="kernel_trace.csv of a run of the same command")
+    q.add_argument("out")
+    q.add_argument("--kernel", action="append", required=True, help="name substring, repeatable")
+    q.add_argument("--grid-size", type=int, action="append", default=[],
+                   help="only dispatches with one of these grid sizes (threads)")
+    q.add_argument("--simds", type=int, default=1024)
+    q.add_argument("--clock-ghz", type=float, default=2.4)
+    q.add_argument("--title", default="")
+    q.add_argument("--cmd", required=True)
+    q.add_argument("--trace-cmd", required=True)
     args = ap.parse_args()
-    {"stats": stats, "traffic": traffic, "trace": trace, "window": window}[args.mode](args)
+    {"stats": stats, "traffic": traffic, "trace": trace, "window": window,
+     "sq": sq}[args.mode](args)
 
 
 if __name__ == "__main__":
