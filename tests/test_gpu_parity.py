@@ -121,6 +121,22 @@ def test_upload_download_roundtrip(P, gpu):
         lat.close()
 
 
+@pytest.mark.parametrize("nx,ny,ld", [(33, 70, 70), (33, 70, 80), (8, 3, 16), (1, 5, 9)])
+def test_pitched_transfers_keep_padding(P, gpu, nx, ny, ld):
+    """Host arrays with pitch ld >= ny (the reference's padded f(ld, nx, 0:8)): upload
+    reads rows 0..ny-1 of each column, download writes them and leaves the padding rows
+    ny..ld-1 as the caller had them, for ld == ny (one copy per plane) and ld > ny."""
+    rng = np.random.default_rng(3)
+    f = np.asfortranarray(rng.random((ld, nx, 9)))
+    lat = P.Lattice(nx, ny, 2, "f64")
+    lat.upload_f(0, f)
+    out = np.full((ld, nx, 9), -7.0, order="F")
+    lat.download_f(0, ld=ld, out=out)
+    lat.close()
+    np.testing.assert_array_equal(out[:ny], f[:ny])
+    assert np.all(out[ny:] == -7.0)
+
+
 # ragged shapes, a single row or column, and a grid larger than one sweep of the 1024
 # reduction workgroups (several cells per thread)
 @pytest.mark.parametrize("nx,ny", [(70, 45), (5, 3), (1, 7), (300, 1), (1000, 530)])
