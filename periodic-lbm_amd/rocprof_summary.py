@@ -5,6 +5,8 @@
          --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py window <kernel_trace.csv> <out.md>
          --sequence <name substring> ... [--bytes B ...] --skip K --cmd "<cmd>"
+  python periodic-lbm_amd/rocprof_summary.py span <kernel_trace.csv> <out.md>
+         --kernel <substring> [--kernel ...] [--cells N --steps K] --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py sq <counter_collection.csv> <kernel_trace.csv>
          <out.md> --kernel <substring> [--kernel ...] --grid-size N --cmd "<pmc cmd>"
          --trace-cmd "<trace cmd>"
@@ -151,6 +153,51 @@ def traffic(args):
     Path(args.out).write_text(json.dumps(rec, indent=1) + "\n")
 
 
+def span(args):
+    """The last uninterrupted run of dispatches whose names contain one of the given
+    substrings, cut where the GPU idled longer than --split-gap-us between two of them
+    (a host synchronisation: e.g. the timed run of the batched engine, whose hipGraph
+    replays only pass kernels back to back): dispatches per kernel, the span from the
+    first start to the last end, the time the GPU was busy with them (union of
+    intervals), and cells x steps / span."""
+    rows = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                  for r in _rows(args.csv))
+    match = [any(k in n for k in args.kernel) for _, _, n in rows]
+    end = max((i for i, m in enumerate(match) if m), default=None)
+    if end is None:
+        raise SystemExit(f"no dispatch of {args.kernel}")
+    start = end
+    gap = args.split_gap_us * 1e3
+    while start > 0 and match[start - 1] and rows[start][0] - rows[start - 1][1] <= gap:
+        start -= 1
+    sel = rows[start:end + 1]
+    per = {}
+    for t0, t1, n in sel:
+        per.setdefault(n, []).append(t1 - t0)
+    t_first, t_last = sel[0][0], max(t1 for _, t1, _ in sel)
+    busy, cur0, cur1 = 0, None, None
+    for t0, t1, _ in sel:
+        if cur1 is None or t0 > cur1:
+            busy += 0 if cur1 is None else cur1 - cur0
+            cur0, cur1 = t0, t1
+        else:
+            cur1 = max(cur1, t1)
+    busy += cur1 - cur0
+    sp = t_last - t_first
+    lines = [f"# rocprofv3 kernel trace: {args.title}", "", f"Command: `{args.cmd}`", "",
+             f"The last uninterrupted run of dispatches of {', '.join(args.kernel)} (no idle "
+             f"gap above {args.split_gap_us:g} us): {len(sel)} dispatches.", "",
+             "| kernel | dispatches | avg us | total ms |", "|---|---:|---:|---:|"]
+    for n, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"| `{n[:60]}` | {len(d)} | {sum(d) / len(d) / 1e3:.1f} | {sum(d) / 1e6:.2f} |")
+    lines += ["", f"Span of these dispatches: {sp / 1e6:.2f} ms, of which kernels busy "
+                  f"{busy / 1e6:.2f} ms (gaps {(sp - busy) / 1e6:.2f} ms)."]
+    if args.cells and args.steps:
+        lines[-1] += (f"  {args.cells:.0f} cells x {args.steps} steps / span = "
+                      f"{args.cells * args.steps / (sp * 1e-9) / 1e6:.0f} MLUPS.")
+    Path(args.out).write_text("\n".join(lines) + "\n")
+
+
 SQ_COUNTERS = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD",
                "SQ_INSTS_VMEM_WR", "SQ_WAVES"]
 
@@ -235,6 +282,15 @@ def main():
     w.add_argument("--skip", type=int, default=0, help="leading dispatches before the window")
     w.add_argument("--title", default="timed region")
     w.add_argument("--cmd", required=True)
+    sp = sub.add_parser("span")
+    sp.add_argument("csv")
+    sp.add_argument("out")
+    sp.add_argument("--kernel", action="append", required=True, help="name substring, repeatable")
+    sp.add_argument("--cells", type=float, default=0.0)
+    sp.add_argument("--steps", type=int, default=0)
+    sp.add_argument("--split-gap-us", type=float, default=50.0)
+    sp.add_argument("--title", default="dispatch span")
+    sp.add_argument("--cmd", required=True)
     q = sub.add_parser("sq")
     q.add_argument("counters", help="counter_collection.csv of a --pmc run of SQ_ counters")
     q.add_argument("trace", help="kernel_trace.csv of a run of the same command")
@@ -249,7 +305,7 @@ def main():
     q.add_argument("--trace-cmd", required=True)
     args = ap.parse_args()
     {"stats": stats, "traffic": traffic, "trace": trace, "window": window,
-     "sq": sq}[args.mode](args)
+     "span": span, "sq": sq}[args.mode](args)
 
 
 if __name__ == "__main__":
