@@ -252,6 +252,8 @@ def test_pass_and_run_errors(P, gpu):
         lat.run_pass(7, 2, 0, 1)
     with pytest.raises(P.PlbmError, match="unknown collision"):
         lat.run(7, 4, 0, 1)
+    with pytest.raises(P.PlbmError, match="nsteps"):
+        lat.run("trt", -1, 0, 1)
     with pytest.raises(P.PlbmError):
         lat.step_op("fvm_bardow", "trt", 0, 1, float("nan"))
     with pytest.raises(P.PlbmError):
@@ -268,3 +270,9 @@ def test_pass_and_run_errors(P, gpu):
     ref.set_steps_per_launch("trt", 1)
     i, _ = ref.run("trt", 2, 0, 1)
     np.testing.assert_array_equal(lat.download_f(1), ref.download_f(i))
+    # zero steps: slots and contents unchanged, at the automatic and a forced depth
+    for depth in (0, 8):
+        lat.set_steps_per_launch("trt", depth)
+        before = lat.download_f(1)
+        assert lat.run("trt", 0, 1, 0) == (1, 0)
+        np.testing.assert_array_equal(lat.download_f(1), before)
