@@ -198,8 +198,13 @@ __global__ void __launch_bounds__(256)
 // in LDS two ahead of the walk.  When level k receives a
 // column, level k+1 gets the column before it: the operator's 3x3 stencil per direction,
 // then the collision, exactly as k_stream_fd<OP, F64, MODEL> computes them.
+// Two wavefronts per SIMD are requested: the 3-step passes of the 9-point Bardow
+// operators with TRT / RR / split BGK need 258-264 registers and would otherwise run one
+// wavefront per SIMD; held to 256 they spill 2-8 registers and run faster (8192^2, ms per
+// step in passes of 3, one / two wavefronts per SIMD: FDM + TRT 1.31 / 1.04, FVM + TRT
+// 1.80 / 1.35).
 template <int OP, int MODEL, int NS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_stream_fd_multi(StepArgs a, int waves_y, int nwaves, double dt, double p2s) {
     constexpr int VR = 56;
     static_assert(VR + 2 * NS <= 64, "halo rows do not fit in a wavefront");

@@ -79,17 +79,17 @@ extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst
 }
 
 // Steps per pass of plbm_run_op when the caller has not chosen, where the grid gives at
-// least 2048 wavefronts (FP64 storage; else single steps): 3 for the Sofonea operator, 2
-// for the 9-point Bardow operators, whose 3-step pass no longer fits two wavefronts per
-// SIMD (measured at 8192^2, ms per step in passes of 1 / 2 / 3: FVM + BGK 2.06 / 1.71 /
-// 1.89, FDM + TRT 2.02 / 1.05 / 1.30, Sofonea + BGK 1.94 / 1.00 / 0.69).
+// least 2048 wavefronts (FP64 storage; else single steps): 3, the most a pass holds
+// (8192^2, ms per step in passes of 2 / 3: FVM + BGK 1.68 / 1.66, FVM + TRT 1.35 / 1.35,
+// FDM + BGK 1.11 / 1.08, FDM + TRT 1.09 / 1.04, Sofonea + BGK 1.00 / 0.69; single steps
+// about 2.0).
 static int fd_steps_per_pass(const plbm_lattice *h, int op, int model) {
+    (void)op;
     (void)model;
     if (h->fd_ns > 0) return h->fd_ns;
     if (h->storage != PLBM_F64) return 1;
     const long waves = (long)((h->ny + 55) / 56) * ((h->nx + kMultiXS - 1) / kMultiXS);
-    if (waves < 2048) return 1;
-    return op == PLBM_STREAM_FDM_SOFONEA ? 3 : 2;
+    return waves < 2048 ? 1 : 3;
 }
 
 extern "C" int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps) {
