@@ -160,9 +160,10 @@ int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt);
 int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt);
 /* nsteps x perform_step with streaming operator `op` and collide_<model> (the fused pair),
  * with the reference's buffer bookkeeping as plbm_run (each step reads *iold, writes
- * *inew, swaps them).  In FP64 storage all steps but the last run in temporally blocked
- * passes of up to plbm_set_op_steps_per_pass steps (bit-identical to single steps; the
- * two slots may exchange their device storage, as in plbm_run).  Asynchronous. */
+ * *inew, swaps them).  In FP64 storage the steps run in temporally blocked passes of up
+ * to plbm_set_op_steps_per_pass steps, the last pass also writing the state one step
+ * back into the lattice's third buffer as plbm_run does (bit-identical to single steps;
+ * the slots may exchange their device storage, as in plbm_run).  Asynchronous. */
 int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *iold, int *inew,
                 double dt);
 /* Pass depth of plbm_run_op: 1 = single fused steps, 2..3 = passes (FP64 storage only),
