@@ -64,7 +64,11 @@ __global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
     if (gw >= waves_y * m) return;  // the whole wavefront
     const int band = gw % waves_y, j = gw / waves_y;
     const int x0 = (int)((long long)a.nx * j / m), x1 = (int)((long long)a.nx * (j + 1) / m);
-    if (x1 > x0) multi_wave<MODEL, ST, NS>(a, band, x0, x1 - x0);
+    // the unrolled steady state where it fits two wavefronts per SIMD and measured faster
+    // (8192^2 FP64 TRT, same-box A/B, ms per pass: 6 steps 2.11 -> 2.03, 7 steps 2.40 ->
+    // 2.26; FP32 storage and BGK measured slower with it, so it stays with FP64 TRT)
+    constexpr bool rot = MODEL == PLBM_TRT && ST == PLBM_F64 && (NS == 6 || NS == 7);
+    if (x1 > x0) multi_wave<MODEL, ST, NS, rot>(a, band, x0, x1 - x0);
 }
 
 int max_steps_per_pass(int) { return kMaxPassSteps; }

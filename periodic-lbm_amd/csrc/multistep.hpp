@@ -175,6 +175,7 @@ struct ColumnStage {
     __device__ __forceinline__ const double *column(int i) const {
         return buf0 + (i % kStageBufs) * kStageSlots * 64;
     }
+
 };
 
 // Lane of a staged slot that lane `lane` reads for the pull in y of a population with
@@ -212,8 +213,9 @@ __device__ __forceinline__ R pull_y(int q, R v) {
 // The calling wavefront advances rows [VR*wy, VR*wy + VR) (clipped to the grid) x columns
 // [x0, x0 + ncols) (0 <= x0, x0 + ncols <= nx) of the grid described by `a` NS steps,
 // a.src -> a.dst, and, when a.lag is set, stores level NS-1 of the same cells there.
-// All 64 lanes must call it together (lane shifts and shuffles).
-template <int MODEL, int ST, int NS>
+// All 64 lanes must call it together (lane shifts and shuffles).  ROT: see the steady
+// state below (the arithmetic and results are the same either way).
+template <int MODEL, int ST, int NS, bool ROT = false>
 __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
     using T = typename Traits<ST>::T;
     using R = typename Traits<ST>::R;
@@ -229,7 +231,8 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     const T *__restrict__ s = static_cast<const T *>(a.src);
     T *__restrict__ d = static_cast<T *>(a.dst);
     T *__restrict__ lag = static_cast<T *>(a.lag);  // wave-uniform
-    R W[NS][3][9];  // [level][newest column, -1, -2][q]
+    // W[k][.][q]: level k's pull-ready values of its three newest columns.
+    R W[NS][3][9];
 #pragma unroll
     for (int k = 0; k < NS; ++k)
 #pragma unroll
@@ -245,6 +248,7 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     // one column into registers otherwise), so that their latency overlaps the
     // collisions of the current column.
     constexpr bool kStage = sizeof(T) == 8;
+    static_assert(!kStage || kStageBufs == 3, "staging buffers rotate with the window");
     T nxt[9];
     ColumnStage st{};
     int pull_lane[3];  // staged slot lanes of the level-0 pull for cy = -1, 0, +1
@@ -267,54 +271,101 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
 #pragma unroll
     for (int j = 0; j < ahead; ++j)
         if (j < iters) fetch(x0 - NS + j, j);
-    for (int i = 0; i < iters; ++i) {
+
+    // Iteration i loads level 0 at relative column i (column x0 - NS + i) and, once
+    // i >= 2k, advances level k (k = 1 .. NS) at column i - k from level k-1's columns
+    // i - k + 1 (the newest), i - k and i - k - 1.  Iterations past the walk's end (at
+    // most two, see the steady state) load and store nothing and only re-read the last
+    // column's staged values.
+    auto load_level0 = [&](int i, int buf, int slot0, int fetch_buf)
+                           __attribute__((always_inline)) {
         if constexpr (kStage) {
-            st.wait(i, iters, 2 * NS, lag != nullptr);
-            const double *slot = st.column(i);
+            if (!ROT || i < iters) st.wait(i, iters, 2 * NS, lag != nullptr);
+            const double *mem = st.buf0 + buf * kStageSlots * 64;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                W[0][2][q] = W[0][1][q];
-                W[0][1][q] = W[0][0][q];
+            for (int q = 0; q < 9; ++q)
                 // the pull in y reads the neighbouring row's slot (edge lanes are halo)
-                W[0][0][q] = load_pop<ST>(
-                    reinterpret_cast<const T *>(slot + q * 64 + pull_lane[cy(q) + 1]), a.inv_scale);
-            }
+                W[0][slot0][q] = load_pop<ST>(
+                    reinterpret_cast<const T *>(mem + q * 64 + pull_lane[cy(q) + 1]), a.inv_scale);
         } else {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                W[0][2][q] = W[0][1][q];
-                W[0][1][q] = W[0][0][q];
-                W[0][0][q] = pull_y(q, load_pop<ST>(&nxt[q], a.inv_scale));
-            }
+            for (int q = 0; q < 9; ++q) W[0][slot0][q] = pull_y(q, load_pop<ST>(&nxt[q], a.inv_scale));
         }
-        if (i + ahead < iters) fetch(x0 - NS + i + ahead, (i + ahead) % (kStage ? kStageBufs : 2));
+        if (i + ahead < iters) fetch(x0 - NS + i + ahead, fetch_buf);
+    };
+    // Level k's collision of iteration i from level k-1's slots b0 (its newest column),
+    // b1 and b2; stores level NS (and level NS-1 into `lag`).  Returns the result in f.
+    auto advance = [&](int i, int k, int b0, int b1, int b2, R (&f)[9])
+                       __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            f[q] = cx(q) > 0 ? W[k - 1][b2][q] : (cx(q) == 0 ? W[k - 1][b1][q] : W[k - 1][b0][q]);
+        collide_cell<MODEL, ST>(f, a);
+        // level NS-1 reaches column x0 + j at iteration 2 NS - 1 + j, level NS at 2 NS + j
+        if (k == NS - 1 && lag != nullptr && i >= 2 * NS - 1 && i < 2 * NS - 1 + ncols && out) {
+            const size_t co = (size_t)(x0 - 2 * NS + 1 + i) * cs;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) store_pop<ST>(lag + co + q * L + y, f[q], a.scale);
+        }
+        if (k == NS && out && (!ROT || i < iters)) {
+            const size_t co = (size_t)(x0 - 2 * NS + i) * cs;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) store_pop<ST>(d + co + q * L + y, f[q], a.scale);
+        }
+    };
+
+    // Start-up (levels switch on one by one), or the whole walk without ROT: slot 0 holds
+    // a level's newest column, 1 and 2 the older ones; a level that advances shifts them.
+    const int shift_end = ROT ? 2 * NS : iters;
+    for (int i = 0; i < shift_end; ++i) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            W[0][2][q] = W[0][1][q];
+            W[0][1][q] = W[0][0][q];
+        }
+        load_level0(i, i % 3, 0, (i + ahead) % 3);
 #pragma unroll
         for (int k = 1; k <= NS; ++k) {
             if (i < 2 * k) break;
             R f[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q)
-                f[q] = cx(q) > 0 ? W[k - 1][2][q] : (cx(q) == 0 ? W[k - 1][1][q] : W[k - 1][0][q]);
-            collide_cell<MODEL, ST>(f, a);
+            advance(i, k, 0, 1, 2, f);
             if (k < NS) {
-                // level NS-1 reaches column x0 + j at iteration 2 NS - 1 + j
-                if (k == NS - 1 && lag != nullptr && i >= 2 * NS - 1 && i < 2 * NS - 1 + ncols &&
-                    out) {
-                    const size_t co = (size_t)(x0 - 2 * NS + 1 + i) * cs;
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) store_pop<ST>(lag + co + q * L + y, f[q], a.scale);
-                }
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
                     W[k][2][q] = W[k][1][q];
                     W[k][1][q] = W[k][0][q];
                     W[k][0][q] = pull_y(q, f[q]);
                 }
-            } else if (out) {
-                const size_t co = (size_t)(x0 - 2 * NS + i) * cs;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) store_pop<ST>(d + co + q * L + y, f[q], a.scale);
             }
+        }
+    }
+    // ROT: the steady state (every level advances) is unrolled by three columns so that
+    // each slot index is a compile-time constant and the windows rotate by renaming
+    // registers instead of moving them (60 FP64 moves per column fewer).  Phase
+    // P = (i - 2 NS) % 3; the start-up leaves (newest, -1, -2) in slots (0, 1, 2), so in
+    // phase P column -back sits in slot (back - P - 1) mod 3 of every level.  The walk is
+    // rounded up to whole groups of three.  It costs registers (FP64 TRT: 206 -> 227 at 6
+    // steps, 230 -> 255 at 7, beyond 256 at 8), so the pass kernel asks for it only where
+    // it still fits two wavefronts per SIMD and measured faster.
+    if constexpr (ROT) {
+        auto steady = [&](int i, auto phase) __attribute__((always_inline)) {
+            constexpr int P = decltype(phase)::value;
+            constexpr int s0 = (3 - P - 1) % 3, s1 = (4 - P - 1) % 3, s2 = (5 - P - 1) % 3;
+            load_level0(i, (2 * NS + P) % 3, s0, (2 * NS + P + ahead) % 3);
+#pragma unroll
+            for (int k = 1; k <= NS; ++k) {
+                R f[9];
+                advance(i, k, s0, s1, s2, f);
+                if (k < NS) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) W[k][s0][q] = pull_y(q, f[q]);
+                }
+            }
+        };
+        for (int i = 2 * NS; i < iters; i += 3) {
+            steady(i, std::integral_constant<int, 0>{});
+            steady(i + 1, std::integral_constant<int, 1>{});
+            steady(i + 2, std::integral_constant<int, 2>{});
         }
     }
 }
