@@ -2,8 +2,8 @@
 
 * 8192^2 (the headline grid), FP64: runs in full temporally blocked passes of 8 steps
   (the pass kernel the bench times, including the last pass that writes the lagged
-  state) and an uneven split, both slots against the C oracle bit for bit; 4096^2 TRT
-  with noise (the FP64 roofline case) for 200 steps.
+  state), uneven splits and the benchmark's 20-step run (6 + 6 + 8), both slots against
+  the C oracle bit for bit; 4096^2 TRT with noise (the FP64 roofline case) for 200 steps.
 * 8192^2 TRT and regularized, 5000 steps, FP32- and FP16-shifted storage against the
   FP64 run (itself bit-identical to the oracle): velocity relative L2 and mass drift.
 * The batched sweep: 256 cases, n from {32 .. 512}, BGK, FP32-shifted storage, 2000 steps
@@ -29,11 +29,14 @@ def headline_fields(P, n, u0=0.05, nu=0.01, seed=7):
     return rho, ux, uy, 1.0 / (3 * nu + 0.5)
 
 
-@pytest.mark.parametrize("model,steps", [("trt", 16), ("rr", 16), ("trt", 17)])
+@pytest.mark.parametrize("model,steps", [("trt", 16), ("rr", 16), ("trt", 17), ("trt", 20),
+                                         ("trt", 15)])
 def test_headline_passes_bit_exact_8192(P, gpu, oracle, model, steps):
     """8192^2 FP64 through plbm_run at the automatic depth (8): 16 steps are two full
-    8-step passes (the second also writes the lagged slot), 17 steps run as 4 + 5 + 8.
-    Latest and lagged slot against the C oracle, bit for bit."""
+    8-step passes (the second also writes the lagged slot), 17 steps run as 4 + 5 + 8,
+    20 steps (the benchmark's run) as 6 + 6 + 8 and 15 as 7 + 8, the TRT passes of 6 and
+    7 steps in their unrolled steady-state form.  Latest and lagged slot against the C
+    oracle, bit for bit."""
     n = 8192
     print(f"case headline {model} {n}^2 {steps} steps", flush=True)
     rho, ux, uy, omega = headline_fields(P, n)
@@ -41,8 +44,11 @@ def test_headline_passes_bit_exact_8192(P, gpu, oracle, model, steps):
     lat.set_params(omega, 0.25)
     assert lat.get_steps_per_launch(model) == 8
     lat.equilibrium_init(0, rho, ux, uy)
+    lat.set_run_timing(True)
     iold, inew = lat.run(model, steps, 0, 1)
     assert (iold, inew) == ((1, 0) if steps % 2 else (0, 1))
+    plan = {15: [7, 8], 16: [8, 8], 17: [4, 5, 8], 20: [6, 6, 8]}[steps]
+    assert [d for d, _ in lat.run_launch_times()] == plan
     got_latest = lat.download_f(iold)
     got_prev = lat.download_f(inew)
     lat.close()
