@@ -34,9 +34,10 @@ extern "C" {
 #define PLBM_ERR_ARG 1      /* invalid argument (size, id, null pointer, model) */
 #define PLBM_ERR_HIP 2      /* a HIP runtime call failed (message has hipGetErrorString) */
 #define PLBM_ERR_STATE 3    /* call not valid in the current state (e.g. no params set) */
-#define PLBM_ERR_NOMEM 4    /* device allocation failed */
+#define PLBM_ERR_NOMEM 4    /* device or host allocation failed */
 #define PLBM_ERR_NUMERIC 5  /* non-finite populations detected by a checked call */
 #define PLBM_ERR_TIMEOUT 6  /* a bounded device-side wait gave up (batched engine) */
+#define PLBM_ERR_HOST 7     /* a host-side failure other than allocation (C++ exception) */
 
 /* Collision models: collide_bgk (src/collision_bgk.F90:17), collide_trt
  * (src/collision_trt.F90:41), collide_rr (src/collision_regularized.F90:21). */

@@ -16,7 +16,7 @@ LIBPATH = PKG / "libplbm.so"
 
 # status codes and enums, include/plbm.h
 OK = 0
-ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_NUMERIC, ERR_TIMEOUT = 1, 2, 3, 4, 5, 6
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_NUMERIC, ERR_TIMEOUT, ERR_HOST = 1, 2, 3, 4, 5, 6, 7
 BGK, TRT, RR, BGK_SPLIT = 0, 1, 2, 3
 F64, F64S, F32S, F16S = 0, 1, 2, 3
 RED_MASS, RED_KE, RED_ERR2, RED_REF2, RED_UMAX, RED_NONFINITE, RED_U2 = range(7)

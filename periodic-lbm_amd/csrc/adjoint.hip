@@ -267,7 +267,7 @@ static void *seg(plbm_adjoint *a, int k) {
 extern "C" {
 
 int plbm_adjoint_create(int nx, int ny, int storage, int nsteps, int checkpoint_every,
-                        plbm_adjoint **out) {
+                        plbm_adjoint **out) try {
     PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output pointer");
     *out = nullptr;
     PLBM_REQUIRE(storage == PLBM_F64 || storage == PLBM_F64S || storage == PLBM_F32S,
@@ -307,9 +307,11 @@ int plbm_adjoint_create(int nx, int ny, int storage, int nsteps, int checkpoint_
         return fail("hipMalloc of adjoint scalars failed");
     *out = a;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_adjoint_destroy(plbm_adjoint *a) {
+int plbm_adjoint_destroy(plbm_adjoint *a) try {
     if (!a) return PLBM_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     if (a->checkpoints) (void)hipFree(a->checkpoints);
@@ -321,18 +323,22 @@ int plbm_adjoint_destroy(plbm_adjoint *a) {
     if (a->lat) plbm_destroy(a->lat);
     delete a;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_adjoint_set_stream(plbm_adjoint *a, void *stream) {
+int plbm_adjoint_set_stream(plbm_adjoint *a, void *stream) try {
     PLBM_REQUIRE(a, PLBM_ERR_ARG, "null adjoint handle");
     PLBM_TRY(plbm_set_stream(a->lat, stream));
     a->stream = a->lat->stream;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_adjoint_forward_dev(plbm_adjoint *a, const double *rho0, const double *ux0,
                              const double *uy0, double omega, const double *uxt,
-                             const double *uyt, double *loss) {
+                             const double *uyt, double *loss) try {
     PLBM_REQUIRE(a && rho0 && ux0 && uy0 && uxt && uyt && loss, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(std::isfinite(omega), PLBM_ERR_ARG, "non-finite omega");
     plbm_lattice *h = a->lat;
@@ -372,10 +378,12 @@ int plbm_adjoint_forward_dev(plbm_adjoint *a, const double *rho0, const double *
     PLBM_HIP(hipStreamSynchronize(a->stream));
     a->forward_done = true;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_adjoint_backward_dev(plbm_adjoint *a, double *grho0, double *gux0, double *guy0,
-                              double *gomega) {
+                              double *gomega) try {
     PLBM_REQUIRE(a && gux0 && guy0 && gomega, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(a->forward_done, PLBM_ERR_STATE, "run plbm_adjoint_forward_dev first");
     plbm_lattice *h = a->lat;
@@ -420,6 +428,8 @@ int plbm_adjoint_backward_dev(plbm_adjoint *a, double *grho0, double *gux0, doub
                             a->stream));
     PLBM_HIP(hipStreamSynchronize(a->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 }  // extern "C"

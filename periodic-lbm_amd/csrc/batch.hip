@@ -472,7 +472,7 @@ static int ensure_btmp(plbm_batch *b, size_t bytes) {
 extern "C" {
 
 int plbm_batch_create(int ncases, const int *nx, const int *ny, int storage, int model,
-                      plbm_batch **out) {
+                      plbm_batch **out) try {
     PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output pointer");
     *out = nullptr;
     PLBM_REQUIRE(ncases >= 1 && nx && ny, PLBM_ERR_ARG, "need ncases >= 1 and size arrays");
@@ -527,9 +527,11 @@ int plbm_batch_create(int ncases, const int *nx, const int *ny, int storage, int
     }
     *out = b;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_destroy(plbm_batch *b) {
+int plbm_batch_destroy(plbm_batch *b) try {
     if (!b) return PLBM_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->side) (void)hipStreamSynchronize(b->side);
@@ -546,9 +548,11 @@ int plbm_batch_destroy(plbm_batch *b) {
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_info(const plbm_batch *b, int *ncases, long long *cells, size_t *arena_bytes) {
+int plbm_batch_info(const plbm_batch *b, int *ncases, long long *cells, size_t *arena_bytes) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     long long n = 0;
     for (const auto &k : b->cases) n += (long long)k.nx * k.ny;
@@ -556,9 +560,11 @@ int plbm_batch_info(const plbm_batch *b, int *ncases, long long *cells, size_t *
     if (cells) *cells = n;
     if (arena_bytes) *arena_bytes = b->arena_elems * storage_bytes(b->storage);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_set_params(plbm_batch *b, const double *omega, const double *trt_magic) {
+int plbm_batch_set_params(plbm_batch *b, const double *omega, const double *trt_magic) try {
     PLBM_REQUIRE(b && omega, PLBM_ERR_ARG, "null argument");
     for (int c = 0; c < b->ncases; ++c) {
         const double m = trt_magic ? trt_magic[c] : 0.25;
@@ -575,10 +581,12 @@ int plbm_batch_set_params(plbm_batch *b, const double *omega, const double *trt_
                        hipMemcpyHostToDevice));
     b->params_set = true;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_batch_equilibrium_init(plbm_batch *b, int c, const double *rho, const double *ux,
-                                const double *uy) {
+                                const double *uy) try {
     PLBM_TRY(check_case(b, c));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     const BatchCase &k = b->cases[c];
@@ -598,9 +606,11 @@ int plbm_batch_equilibrium_init(plbm_batch *b, int c, const double *rho, const d
     }));
     PLBM_HIP(hipStreamSynchronize(b->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld) {
+int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld) try {
     PLBM_TRY(check_case(b, c));
     const BatchCase &k = b->cases[c];
     PLBM_REQUIRE(f && ld >= k.ny, PLBM_ERR_ARG, "null array or ld < ny");
@@ -621,9 +631,11 @@ int plbm_batch_download_f(plbm_batch *b, int c, double *f, int ld) {
                                stage));
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_upload_f(plbm_batch *b, int c, const double *f, int ld) {
+int plbm_batch_upload_f(plbm_batch *b, int c, const double *f, int ld) try {
     PLBM_TRY(check_case(b, c));
     const BatchCase &k = b->cases[c];
     PLBM_REQUIRE(f && ld >= k.ny, PLBM_ERR_ARG, "null array or ld < ny");
@@ -645,9 +657,11 @@ int plbm_batch_upload_f(plbm_batch *b, int c, const double *f, int ld) {
         PLBM_HIP(hipStreamSynchronize(b->stream));
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_macros(plbm_batch *b, int c, double *rho, double *ux, double *uy) {
+int plbm_batch_macros(plbm_batch *b, int c, double *rho, double *ux, double *uy) try {
     PLBM_TRY(check_case(b, c));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     const BatchCase &k = b->cases[c];
@@ -663,9 +677,11 @@ int plbm_batch_macros(plbm_batch *b, int c, double *rho, double *ux, double *uy)
         uy[i] = (((v[5] - v[7]) + (v[6] - v[8])) + (v[2] - v[4])) / r;
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_reduce(plbm_batch *b, double *out) {
+int plbm_batch_reduce(plbm_batch *b, double *out) try {
     PLBM_REQUIRE(b && out, PLBM_ERR_ARG, "null argument");
     PLBM_TRY(dispatch_storage(b->storage, [&](auto S) {
         hipLaunchKernelGGL(k_batch_reduce<decltype(S)::value>, dim3(b->ncases), dim3(kRedThreads), 0,
@@ -678,9 +694,11 @@ int plbm_batch_reduce(plbm_batch *b, double *out) {
                             hipMemcpyDeviceToHost, b->stream));
     PLBM_HIP(hipStreamSynchronize(b->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int resident_block) {
+int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int resident_block) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(ty >= 1 && tx >= 1 && ty * tx <= 1024 && (ty * tx) % 64 == 0, PLBM_ERR_ARG,
                  "tile must have ty*tx a multiple of 64 and at most 1024 threads");
@@ -694,9 +712,11 @@ int plbm_batch_set_launch(plbm_batch *b, int ty, int tx, int resident_cells, int
     c.resident_cells = resident_cells;
     c.resident_block = resident_block;
     return set_cfg(b, c);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps) {
+int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(steps >= 1 && steps <= max_steps_per_pass(b->storage), PLBM_ERR_ARG,
                  "steps per pass must be 1.." + std::to_string(max_steps_per_pass(b->storage)) +
@@ -705,40 +725,50 @@ int plbm_batch_set_steps_per_pass(plbm_batch *b, int steps) {
     BatchCfg c = b->cfg;
     c.ns = steps;
     return set_cfg(b, c);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps) {
+int plbm_batch_get_steps_per_pass(const plbm_batch *b, int *steps) try {
     PLBM_REQUIRE(b && steps, PLBM_ERR_ARG, "null argument");
     *steps = b->cfg.ns;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_set_pass_columns(plbm_batch *b, int columns) {
+int plbm_batch_set_pass_columns(plbm_batch *b, int columns) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(columns >= 1, PLBM_ERR_ARG, "columns per pass segment must be >= 1");
     PLBM_HIP(hipStreamSynchronize(b->stream));
     BatchCfg c = b->cfg;
     c.xs = columns;
     return set_cfg(b, c);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns) {
+int plbm_batch_get_pass_columns(const plbm_batch *b, int *columns) try {
     PLBM_REQUIRE(b && columns, PLBM_ERR_ARG, "null argument");
     *columns = b->cfg.xs;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_batch_get_launch(const plbm_batch *b, int *ty, int *tx, int *resident_cells,
-                          int *resident_block) {
+                          int *resident_block) try {
     PLBM_REQUIRE(b && ty && tx && resident_cells && resident_block, PLBM_ERR_ARG, "null argument");
     *ty = b->cfg.ty;
     *tx = b->cfg.tx;
     *resident_cells = b->cfg.resident_cells;
     *resident_block = b->cfg.resident_block;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_run(plbm_batch *b, int nsteps) {
+int plbm_batch_run(plbm_batch *b, int nsteps) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
     PLBM_REQUIRE(b->params_set, PLBM_ERR_STATE, "call plbm_batch_set_params first");
@@ -757,9 +787,11 @@ int plbm_batch_run(plbm_batch *b, int nsteps) {
         done += n;
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_run_direct(plbm_batch *b, int nsteps) {
+int plbm_batch_run_direct(plbm_batch *b, int nsteps) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
     PLBM_REQUIRE(b->params_set, PLBM_ERR_STATE, "call plbm_batch_set_params first");
@@ -768,18 +800,22 @@ int plbm_batch_run_direct(plbm_batch *b, int nsteps) {
                            b->cfg, b->parity, nsteps));
     b->parity = (b->parity + nsteps) & 1;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_batch_sync(plbm_batch *b) {
+int plbm_batch_sync(plbm_batch *b) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_HIP(hipStreamSynchronize(b->stream));
     PLBM_HIP(hipStreamSynchronize(b->side));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 // Autotuner: time `steps` steps of every candidate (tile shape x resident threshold) on a
 // scratch copy of the arena (the batch's state is untouched), keep the fastest.
-int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
+int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) try {
     PLBM_REQUIRE(b, PLBM_ERR_ARG, "null batch handle");
     PLBM_REQUIRE(steps >= 2 && steps % 2 == 0, PLBM_ERR_ARG, "steps must be even and >= 2");
     PLBM_REQUIRE(b->params_set, PLBM_ERR_STATE, "call plbm_batch_set_params first");
@@ -866,6 +902,8 @@ int plbm_batch_autotune(plbm_batch *b, int steps, double *best_ms) {
     PLBM_TRY(set_cfg(b, best));
     if (best_ms) *best_ms = tbest;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 }  // extern "C"

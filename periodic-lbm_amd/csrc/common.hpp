@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -18,6 +20,11 @@ namespace plbm {
 // Last error text of the calling thread (plbm_last_error).
 void set_error(const std::string &msg);
 const char *last_error();
+// The handler of the C ABI's int-returning entry points, which are function-try-blocks:
+// no C++ exception (a failed host allocation, say) unwinds into a C, Fortran or Python
+// caller.  Call it only inside a catch clause; it returns PLBM_ERR_NOMEM for an
+// allocation failure, PLBM_ERR_HOST otherwise, and sets the error text.
+int host_exception() noexcept;
 
 #define PLBM_HIP(call)                                                                    \
     do {                                                                                  \

@@ -369,7 +369,7 @@ static void launch_flux_faces(plbm_lattice *h, const double *ft, const double *f
 using namespace plbm;
 
 extern "C" int plbm_dugks_stream_faces(plbm_lattice *h, int src, int dst, double dt,
-                                       double omega_face) {
+                                       double omega_face) try {
     PLBM_TRY(dugks_check(h, src, dst));
     PLBM_REQUIRE(std::isfinite(dt) && std::isfinite(omega_face), PLBM_ERR_ARG,
                  "non-finite dt or face relaxation rate");
@@ -377,11 +377,13 @@ extern "C" int plbm_dugks_stream_faces(plbm_lattice *h, int src, int dst, double
                              static_cast<double *>(h->f[dst]), dt, 0.0, omega_face);
     PLBM_HIP(hipGetLastError());
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_dugks_step_faces(plbm_lattice *h, int iold, int inew, int nsteps,
                                      double dt, double omega_full, double omega_half,
-                                     double omega_face) {
+                                     double omega_face) try {
     PLBM_TRY(dugks_check(h, iold, inew));
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "negative step count");
     PLBM_REQUIRE(std::isfinite(dt) && std::isfinite(omega_full) && std::isfinite(omega_half) &&
@@ -399,10 +401,12 @@ extern "C" int plbm_dugks_step_faces(plbm_lattice *h, int iold, int inew, int ns
     const int want = (nsteps % 2) ? inew : iold;
     if (nsteps > 0 && h->f[want] != latest) std::swap(h->f[iold], h->f[inew]);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omega_full,
-                                  double omega_half) {
+                                  double omega_half) try {
     PLBM_TRY(dugks_check(h, src, dst));
     PLBM_REQUIRE(std::isfinite(omega_full) && std::isfinite(omega_half), PLBM_ERR_ARG,
                  "non-finite relaxation rate");
@@ -411,15 +415,19 @@ extern "C" int plbm_dugks_collide(plbm_lattice *h, int src, int dst, double omeg
                          omega_full, omega_half);
     PLBM_HIP(hipGetLastError());
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_dugks_stream(plbm_lattice *h, int src, int dst, double dt) {
+extern "C" int plbm_dugks_stream(plbm_lattice *h, int src, int dst, double dt) try {
     PLBM_TRY(dugks_check(h, src, dst));
     PLBM_REQUIRE(std::isfinite(dt), PLBM_ERR_ARG, "non-finite dt");
     launch_flux<false>(h, static_cast<const double *>(h->f[src]), nullptr,
                        static_cast<double *>(h->f[dst]), dt, 0.0);
     PLBM_HIP(hipGetLastError());
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 namespace plbm {
@@ -457,21 +465,25 @@ static int dugks_steps_per_pass(const plbm_lattice *h) {
 
 }  // namespace plbm
 
-extern "C" int plbm_dugks_set_steps_per_pass(plbm_lattice *h, int steps) {
+extern "C" int plbm_dugks_set_steps_per_pass(plbm_lattice *h, int steps) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(steps >= 0 && steps <= 3, PLBM_ERR_ARG, "DUGKS steps per pass must be 0..3");
     h->dugks_ns = steps;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_dugks_get_steps_per_pass(const plbm_lattice *h, int *steps) {
+extern "C" int plbm_dugks_get_steps_per_pass(const plbm_lattice *h, int *steps) try {
     PLBM_REQUIRE(h && steps, PLBM_ERR_ARG, "null argument");
     *steps = dugks_steps_per_pass(h);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, double dt,
-                               double omega_full, double omega_half) {
+                               double omega_full, double omega_half) try {
     PLBM_TRY(dugks_check(h, iold, inew));
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "negative step count");
     PLBM_REQUIRE(std::isfinite(dt) && std::isfinite(omega_full) && std::isfinite(omega_half),
@@ -501,4 +513,6 @@ extern "C" int plbm_dugks_step(plbm_lattice *h, int iold, int inew, int nsteps, 
     const int want = (nsteps % 2) ? inew : iold;
     if (h->f[want] != latest) std::swap(h->f[iold], h->f[inew]);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }

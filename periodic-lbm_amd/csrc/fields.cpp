@@ -10,13 +10,17 @@
 
 #include "../../include/plbm.h"
 
+namespace plbm {
+int host_exception() noexcept;  // lattice.hip
+}
+
 extern "C" {
 
 // taylor_green_t constructor + eval, src/benchmarks/taylor_green.f90:31-84.
 // Cell centres are (x-1)+1/2 (:67,69).  The reference's inner loop runs y = 1..nx (:68),
 // which is the same as 1..ny on the square grids it is used with; here y runs over ny.
 int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, double nu,
-                            double t, double *p, double *ux, double *uy) {
+                            double t, double *p, double *ux, double *uy) try {
     if (nx < 1 || ny < 1 || !p || !ux || !uy || !(kx > 0) || !(ky > 0) || !(nu > 0))
         return PLBM_ERR_ARG;
     const double td = 1.0 / (nu * (kx * kx + ky * ky));
@@ -37,11 +41,13 @@ int plbm_field_taylor_green(int nx, int ny, double kx, double ky, double umax, d
         }
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 // eval_shear_layer_grid, src/benchmarks/shear_layer.f90:25-52.
 int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, double delta,
-                           double *rho, double *ux, double *uy) {
+                           double *rho, double *ux, double *uy) try {
     if (nx < 1 || ny < 1 || !rho || !ux || !uy || !(length > 0)) return PLBM_ERR_ARG;
     const double pi = 4.0 * std::atan(1.0);
     #pragma omp parallel for schedule(static)
@@ -57,13 +63,15 @@ int plbm_field_shear_layer(int nx, int ny, double length, double u0, double k, d
         }
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 // eval_vortex_case, src/benchmarks/barotropic_vortex_case.F90:34-83 (Wissocq et al. 2020
 // convected vortex with barotropic density, Eqs. (3), (4), (20)).
 int plbm_field_barotropic_vortex(int nx, int ny, double u0, double xc, double yc, double rc,
                                  double eps, double rho0, double csqr, double *rho, double *ux,
-                                 double *uy) {
+                                 double *uy) try {
     if (nx < 1 || ny < 1 || !rho || !ux || !uy || !(rc > 0) || !(csqr > 0)) return PLBM_ERR_ARG;
     const double half = 1.0 / 2.0;
     const double rcsqr = rc * rc;
@@ -83,6 +91,8 @@ int plbm_field_barotropic_vortex(int nx, int ny, double u0, double xc, double yc
         }
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 // Counter-based uniform deviate in [0,1): splitmix64's output function applied to
@@ -101,7 +111,7 @@ double plbm_noise(uint64_t seed, uint64_t counter) {
 // phi_k = 2 pi U with U the generator's deviates of `seed` (one per mode, in the order
 // the modes are enumerated: m from -kmax to kmax, n from 0 to kmax).
 int plbm_field_band_limited(int nx, int ny, int kmax, double amp, uint64_t seed, double *ux,
-                            double *uy) {
+                            double *uy) try {
     if (nx < 1 || ny < 1 || kmax < 1 || !ux || !uy) return PLBM_ERR_ARG;
     const double pi = 4.0 * std::atan(1.0);
     struct Mode { double kx, ky, a, phi; };
@@ -140,9 +150,11 @@ int plbm_field_band_limited(int nx, int ny, int kmax, double amp, uint64_t seed,
         uy[c] *= s;
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy) {
+int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *uy) try {
     if (n < 0 || !ux || !uy) return PLBM_ERR_ARG;
     #pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < n; ++c) {
@@ -150,6 +162,8 @@ int plbm_field_add_noise(int n, uint64_t seed, double amp, double *ux, double *u
         uy[c] += amp * (2.0 * plbm_noise(seed, 2 * (uint64_t)c + 1) - 1.0);
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 }  // extern "C"

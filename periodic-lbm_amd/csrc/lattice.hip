@@ -30,6 +30,26 @@ namespace plbm {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 const char *last_error() { return g_err.c_str(); }
+int host_exception() noexcept {
+    const char *msg = "host-side C++ exception";
+    int code = PLBM_ERR_HOST;
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        msg = "host memory allocation failed";
+        code = PLBM_ERR_NOMEM;
+    } catch (const std::length_error &) {
+        msg = "host allocation size out of range";
+        code = PLBM_ERR_NOMEM;
+    } catch (...) {
+    }
+    try {
+        g_err = msg;
+    } catch (...) {
+        g_err.clear();
+    }
+    return code;
+}
 
 // ---------------------------------------------------------------------------- helpers
 
@@ -521,7 +541,7 @@ double plbm_magic_number(double le, double ld) {
     return ((2.0 - le) * (2.0 - ld)) / ((4.0 * le) * ld);
 }
 
-int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) {
+int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) try {
     PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output pointer");
     *out = nullptr;
     PLBM_REQUIRE(nx >= 1 && ny >= 1, PLBM_ERR_ARG, "grid size must be positive");
@@ -558,9 +578,11 @@ int plbm_create(int nx, int ny, int nf, int storage, plbm_lattice **out) {
     }
     *out = h;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_destroy(plbm_lattice *h) {
+int plbm_destroy(plbm_lattice *h) try {
     if (!h) return PLBM_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (int b = 0; b < kMaxBuffers; ++b)
@@ -573,9 +595,11 @@ int plbm_destroy(plbm_lattice *h) {
     if (h->own) (void)hipStreamDestroy(h->own);
     delete h;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_get_info(const plbm_lattice *h, plbm_info *out) {
+int plbm_get_info(const plbm_lattice *h, plbm_info *out) try {
     PLBM_REQUIRE(h && out, PLBM_ERR_ARG, "null argument");
     out->nx = h->nx;
     out->ny = h->ny;
@@ -589,22 +613,28 @@ int plbm_get_info(const plbm_lattice *h, plbm_info *out) {
     out->f16_scale = h->scale;
     out->bytes_per_buffer = h->elems * storage_bytes(h->storage);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_set_stream(plbm_lattice *h, void *stream) {
+int plbm_set_stream(plbm_lattice *h, void *stream) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     h->stream = stream ? static_cast<hipStream_t>(stream) : h->own;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_buffer_ptr(plbm_lattice *h, int buf, void **dptr) {
+int plbm_buffer_ptr(plbm_lattice *h, int buf, void **dptr) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(dptr, PLBM_ERR_ARG, "null output pointer");
     *dptr = h->f[buf];
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_set_params(plbm_lattice *h, double omega, double trt_magic) {
+int plbm_set_params(plbm_lattice *h, double omega, double trt_magic) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(std::isfinite(omega) && std::isfinite(trt_magic), PLBM_ERR_ARG,
                  "non-finite collision parameter");
@@ -613,18 +643,22 @@ int plbm_set_params(plbm_lattice *h, double omega, double trt_magic) {
     h->lambda_odd = plbm_lambda_d(omega, trt_magic);
     h->params_set = true;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_set_f16_scale(plbm_lattice *h, double scale) {
+int plbm_set_f16_scale(plbm_lattice *h, double scale) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     int e = 0;
     PLBM_REQUIRE(scale > 0 && std::frexp(scale, &e) == 0.5, PLBM_ERR_ARG,
                  "FP16 scale must be a positive power of two");
     h->scale = h->storage == PLBM_F16S ? scale : 1.0;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) {
+int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
@@ -645,9 +679,11 @@ int plbm_upload_f(plbm_lattice *h, int buf, const double *f, int ld) {
         PLBM_HIP(hipStreamSynchronize(h->stream));
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
+int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(f && ld >= h->ny, PLBM_ERR_ARG, "null array or ld < ny");
     const size_t plane_host = (size_t)ld * h->nx;
@@ -666,10 +702,12 @@ int plbm_download_f(plbm_lattice *h, int buf, double *f, int ld) {
         PLBM_TRY(plane_to_host(f + q * plane_host, ld, h->d_tmp, h->ny, h->nx, h->stream, stage));
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const double *ux,
-                              const double *uy) {
+                              const double *uy) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     return dispatch_storage(h->storage, [&](auto S) {
@@ -679,10 +717,12 @@ int plbm_equilibrium_init_dev(plbm_lattice *h, int buf, const double *rho, const
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
     });
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_equilibrium_init(plbm_lattice *h, int buf, const double *rho, const double *ux,
-                          const double *uy) {
+                          const double *uy) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     const size_t n = (size_t)h->nx * h->ny;
@@ -693,17 +733,21 @@ int plbm_equilibrium_init(plbm_lattice *h, int buf, const double *rho, const dou
     PLBM_TRY(plbm_equilibrium_init_dev(h, buf, h->d_tmp, h->d_tmp + n, h->d_tmp + 2 * n));
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_step(plbm_lattice *h, int model, int src, int dst) {
+int plbm_step(plbm_lattice *h, int model, int src, int dst) try {
     PLBM_TRY(check_buf(h, src));
     PLBM_TRY(check_buf(h, dst));
     PLBM_REQUIRE(src != dst, PLBM_ERR_ARG, "fused step needs distinct src and dst buffers");
     PLBM_TRY(require_params(h));
     return launch_step(h, model, h->f[src], h->f[dst]);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) {
+int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) try {
     PLBM_REQUIRE(h && iold && inew, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
@@ -714,9 +758,11 @@ int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) {
     PLBM_TRY(require_params(h));
     const int ns = h->cfg[model].ns > 0 ? h->cfg[model].ns : default_steps_per_launch(h);
     return run_steps(h, model, ns, nsteps, iold, inew);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_stream(plbm_lattice *h, int src, int dst) {
+int plbm_stream(plbm_lattice *h, int src, int dst) try {
     PLBM_TRY(check_buf(h, src));
     PLBM_TRY(check_buf(h, dst));
     PLBM_REQUIRE(src != dst, PLBM_ERR_ARG, "streaming needs distinct src and dst buffers");
@@ -728,9 +774,11 @@ int plbm_stream(plbm_lattice *h, int src, int dst) {
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
     });
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_collide(plbm_lattice *h, int model, int buf) {
+int plbm_collide(plbm_lattice *h, int model, int buf) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_TRY(require_params(h));
     StepArgs a = make_args(h, h->f[buf], h->f[buf]);
@@ -744,18 +792,22 @@ int plbm_collide(plbm_lattice *h, int model, int buf) {
             return PLBM_OK;
         });
     });
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_copy(plbm_lattice *h, int src, int dst) {
+int plbm_copy(plbm_lattice *h, int src, int dst) try {
     PLBM_TRY(check_buf(h, src));
     PLBM_TRY(check_buf(h, dst));
     if (src == dst) return PLBM_OK;
     PLBM_HIP(hipMemcpyAsync(h->f[dst], h->f[src], h->elems * storage_bytes(h->storage),
                             hipMemcpyDeviceToDevice, h->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) {
+int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     return dispatch_storage(h->storage, [&](auto S) {
@@ -765,9 +817,11 @@ int plbm_macros_dev(plbm_lattice *h, int buf, double *rho, double *ux, double *u
         PLBM_HIP(hipGetLastError());
         return PLBM_OK;
     });
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_macros(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) {
+int plbm_macros(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(rho && ux && uy, PLBM_ERR_ARG, "null field pointer");
     const size_t n = (size_t)h->nx * h->ny;
@@ -778,10 +832,12 @@ int plbm_macros(plbm_lattice *h, int buf, double *rho, double *ux, double *uy) {
     PLBM_HIP(hipMemcpyAsync(uy, h->d_tmp + 2 * n, n * 8, hipMemcpyDeviceToHost, h->stream));
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *uya,
-                    double out[PLBM_NRED]) {
+                    double out[PLBM_NRED]) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(out, PLBM_ERR_ARG, "null output");
     PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
@@ -792,10 +848,12 @@ int plbm_reduce_dev(plbm_lattice *h, int buf, const double *uxa, const double *u
     PLBM_HIP(hipMemcpyAsync(out, h->d_red, PLBM_NRED * 8, hipMemcpyDeviceToHost, h->stream));
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_reduce_static(plbm_lattice *h, int buf, const double *uxa, const double *uya,
-                       double *out_dev) {
+                       double *out_dev) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE(out_dev, PLBM_ERR_ARG, "null output");
     PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
@@ -808,10 +866,12 @@ int plbm_reduce_static(plbm_lattice *h, int buf, const double *uxa, const double
     double *parts = nullptr;
     PLBM_TRY(static_reduce_scratch(h->device, &parts));
     return launch_reduce(h, buf, uxa, uya, parts, out_dev);
+} catch (...) {
+    return plbm::host_exception();
 }
 
 int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
-                double out[PLBM_NRED]) {
+                double out[PLBM_NRED]) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_REQUIRE((uxa == nullptr) == (uya == nullptr), PLBM_ERR_ARG,
                  "uxa and uya must both be given or both be NULL");
@@ -821,15 +881,19 @@ int plbm_reduce(plbm_lattice *h, int buf, const double *uxa, const double *uya,
     PLBM_HIP(hipMemcpyAsync(h->d_tmp, uxa, n * 8, hipMemcpyHostToDevice, h->stream));
     PLBM_HIP(hipMemcpyAsync(h->d_tmp + n, uya, n * 8, hipMemcpyHostToDevice, h->stream));
     return plbm_reduce_dev(h, buf, h->d_tmp, h->d_tmp + n, out);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_sync(plbm_lattice *h) {
+int plbm_sync(plbm_lattice *h) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_HIP(hipStreamSynchronize(h->stream));
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
+int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     LaunchCfg c{block, xcd_remap, h->cfg[model].ns};
@@ -837,9 +901,11 @@ int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) {
                  "block must be a multiple of 64 in [64,1024] and xcd_remap 0 or 1");
     h->cfg[model] = c;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_set_steps_per_launch(plbm_lattice *h, int model, int ns) {
+int plbm_set_steps_per_launch(plbm_lattice *h, int model, int ns) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     PLBM_REQUIRE(ns >= 0 && ns <= max_steps_per_pass(h->storage), PLBM_ERR_ARG,
@@ -847,24 +913,30 @@ int plbm_set_steps_per_launch(plbm_lattice *h, int model, int ns) {
                      " for this storage");
     h->cfg[model].ns = ns;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *ns) {
+int plbm_get_steps_per_launch(const plbm_lattice *h, int model, int *ns) try {
     PLBM_REQUIRE(h && ns, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     *ns = h->cfg[model].ns > 0 ? h->cfg[model].ns : default_steps_per_launch(h);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap) {
+int plbm_get_launch(const plbm_lattice *h, int model, int *block, int *xcd_remap) try {
     PLBM_REQUIRE(h && block && xcd_remap, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
     *block = h->cfg[model].block;
     *xcd_remap = h->cfg[model].remap;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms) {
+int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms) try {
     PLBM_TRY(check_buf(h, buf));
     PLBM_TRY(require_params(h));
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
@@ -875,9 +947,11 @@ int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms
     h->cfg[model] = best;
     if (best_ms) *best_ms = ms;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
+int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) try {
     PLBM_REQUIRE(ms && iters > 0 && bytes >= 16, PLBM_ERR_ARG, "bad argument");
     const size_t n = bytes / 16;
     PLBM_REQUIRE(n / 256 < (size_t)1 << 31, PLBM_ERR_ARG, "copy probe size too large");
@@ -909,6 +983,8 @@ int plbm_copy_bandwidth(size_t bytes, int iters, double *ms) {
     PLBM_HIP(hipEventElapsedTime(&t, e0, e1));
     *ms = double(t) / iters;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 }  // extern "C"

@@ -271,7 +271,7 @@ int default_steps_per_launch(const plbm_lattice *h) {
 
 using namespace plbm;
 
-extern "C" int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst) {
+extern "C" int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
@@ -284,16 +284,20 @@ extern "C" int plbm_pass(plbm_lattice *h, int model, int steps, int src, int dst
                  "collision parameters not set: call plbm_set_params first");
     if (steps == 1) return launch_step(h, model, h->f[src], h->f[dst]);
     return launch_multi(h, model, steps, h->f[src], h->f[dst]);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_set_pass_segments(plbm_lattice *h, int segments) {
+extern "C" int plbm_set_pass_segments(plbm_lattice *h, int segments) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(segments >= 0, PLBM_ERR_ARG, "segments must be >= 0 (0 = automatic)");
     h->pass_segs = segments;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable) {
+extern "C" int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(enable == 0 || enable == 1, PLBM_ERR_ARG, "enable must be 0 or 1");
     h->aux_enabled = enable != 0;
@@ -304,18 +308,22 @@ extern "C" int plbm_set_pass_lag_buffer(plbm_lattice *h, int enable) {
     }
     if (h->aux_enabled) h->aux_failed = false;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_set_run_timing(plbm_lattice *h, int enable) {
+extern "C" int plbm_set_run_timing(plbm_lattice *h, int enable) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(enable == 0 || enable == 1, PLBM_ERR_ARG, "enable must be 0 or 1");
     h->run_timing = enable != 0;
     h->run_depths.clear();
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_run_launch_times(plbm_lattice *h, int max_launches, int *depths, float *ms,
-                                     int *nlaunches) {
+                                     int *nlaunches) try {
     PLBM_REQUIRE(h && nlaunches && max_launches >= 0, PLBM_ERR_ARG, "bad argument");
     PLBM_REQUIRE(h->run_timing, PLBM_ERR_STATE, "run timing is off (plbm_set_run_timing)");
     const int n = (int)h->run_depths.size();
@@ -328,10 +336,14 @@ extern "C" int plbm_run_launch_times(plbm_lattice *h, int max_launches, int *dep
                                              h->run_events[(size_t)k + 1]));
     }
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_get_pass_segments(const plbm_lattice *h, int *segments) {
+extern "C" int plbm_get_pass_segments(const plbm_lattice *h, int *segments) try {
     PLBM_REQUIRE(h && segments, PLBM_ERR_ARG, "null argument");
     *segments = h->pass_segs;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }

@@ -20,6 +20,8 @@
 // after every step (collision conserves rho and j, and is the identity on feq).
 #include <cmath>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <vector>
 
 #include "../../include/plbm.h"
@@ -57,7 +59,11 @@ void *c_plbm_init(int nx, int ny, double dt, const double *p, const double *u,
         return nullptr;
     }
     const int storage = params ? *static_cast<const int *>(params) : PLBM_F64S;
-    auto *s = new SimPlugin();
+    auto *s = new (std::nothrow) SimPlugin();
+    if (!s) {
+        std::fprintf(stderr, "[c_plbm] init: host memory allocation failed\n");
+        return nullptr;
+    }
     s->nx = nx;
     s->ny = ny;
     if (plbm_create(nx, ny, 2, storage, &s->h) != PLBM_OK) {
@@ -66,9 +72,16 @@ void *c_plbm_init(int nx, int ny, double dt, const double *p, const double *u,
         return nullptr;
     }
     const size_t n = (size_t)nx * ny;
-    s->a.resize(n);
-    s->b.resize(n);
-    s->c.resize(n);
+    try {
+        s->a.resize(n);
+        s->b.resize(n);
+        s->c.resize(n);
+    } catch (const std::exception &) {  // no exception leaves the plugin's C interface
+        std::fprintf(stderr, "[c_plbm] init: host memory allocation failed\n");
+        plbm_destroy(s->h);
+        delete s;
+        return nullptr;
+    }
     const double csqr = 1.0 / 3.0;
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {

@@ -63,19 +63,23 @@ int check_op_args(plbm_lattice *h, int op, int src, int dst, double dt) {
 
 }  // namespace
 
-extern "C" int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt) {
+extern "C" int plbm_stream_op(plbm_lattice *h, int op, int src, int dst, double dt) try {
     PLBM_TRY(check_op_args(h, op, src, dst, dt));
     if (op == PLBM_STREAM_EXACT) return plbm_stream(h, src, dst);
     return launch_fd(h, op, -1, h->f[src], h->f[dst], dt);
+} catch (...) {
+    return plbm::host_exception();
 }
 
-extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt) {
+extern "C" int plbm_step_op(plbm_lattice *h, int op, int model, int src, int dst, double dt) try {
     PLBM_TRY(check_op_args(h, op, src, dst, dt));
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG,
                  "unknown collision model " + std::to_string(model));
     PLBM_REQUIRE(h->params_set, PLBM_ERR_STATE, "plbm_set_params has not been called");
     if (op == PLBM_STREAM_EXACT) return plbm_step(h, model, src, dst);
     return launch_fd(h, op, model, h->f[src], h->f[dst], dt);
+} catch (...) {
+    return plbm::host_exception();
 }
 
 // Steps per pass of plbm_run_op when the caller has not chosen, where the grid gives at
@@ -92,24 +96,28 @@ static int fd_steps_per_pass(const plbm_lattice *h, int op, int model) {
     return waves < 2048 ? 1 : 3;
 }
 
-extern "C" int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps) {
+extern "C" int plbm_set_op_steps_per_pass(plbm_lattice *h, int steps) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(steps >= 0 && steps <= 3, PLBM_ERR_ARG, "steps per pass must be 0..3");
     PLBM_REQUIRE(steps <= 1 || h->storage == PLBM_F64, PLBM_ERR_STATE,
                  "passes of the streaming operators need FP64 (PLBM_F64) storage");
     h->fd_ns = steps;
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_get_op_steps_per_pass(const plbm_lattice *h, int op, int model,
-                                          int *steps) {
+                                          int *steps) try {
     PLBM_REQUIRE(h && steps, PLBM_ERR_ARG, "null argument");
     *steps = fd_steps_per_pass(h, op, model);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
 
 extern "C" int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *iold,
-                           int *inew, double dt) {
+                           int *inew, double dt) try {
     PLBM_REQUIRE(h && iold && inew, PLBM_ERR_ARG, "null argument");
     PLBM_REQUIRE(nsteps >= 0, PLBM_ERR_ARG, "nsteps must be >= 0");
     PLBM_TRY(check_op_args(h, op, *iold, *inew, dt));
@@ -149,4 +157,6 @@ extern "C" int plbm_run_op(plbm_lattice *h, int op, int model, int nsteps, int *
     PLBM_TRY(launch_fd(h, op, model, latest, other, dt));
     finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
     return PLBM_OK;
+} catch (...) {
+    return plbm::host_exception();
 }
