@@ -314,8 +314,12 @@ int plbm_run_launch_times(plbm_lattice *h, int max_launches, int *depths, float 
 /* Autotuner: time every launch shape of the fused step for `model` (reps launches from
  * buffer buf into a scratch buffer; the lattice's buffers are not modified), then passes
  * of 2..8 steps against it per lattice update; keep the fastest for this lattice and
- * cache it per (device, storage, model, grid) for the process.  *best_ms (may be NULL)
- * receives the winner's time per lattice-update step. */
+ * cache it per (device, storage, model, grid) for the process.  With passes, it also
+ * times the pass that ends a run (the one that also stores the lagged state, into the
+ * lattice's third buffer, allocated here if needed) at every depth from the chosen one
+ * up, and keeps for it the depth that makes runs cheapest; plbm_set_steps_per_launch
+ * resets that choice to the depth it sets.  *best_ms (may be NULL) receives the winner's
+ * time per lattice-update step. */
 int plbm_autotune(plbm_lattice *h, int model, int buf, int reps, double *best_ms);
 
 /* ---------------------------------------------------------------- batched engine */

@@ -105,6 +105,8 @@ struct LaunchCfg {
     int remap = 1;
     int ns = 0;  // steps per pass of plbm_run (1 = single-step kernel, 2..8 = temporal
                  // blocking, 0 = chosen from the grid size)
+    int lag_ns = 0;  // steps of a run's last pass, which also stores the lagged state
+                     // (0 = ns; set by the autotuner)
 };
 
 // Per-launch scalars of one step, passed by value.
@@ -210,7 +212,7 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
 // lattice's third buffer (nullptr when it cannot be allocated), and the final assignment
 // of device storage to slots.
 std::vector<int> pass_plan(int nsteps, int ns);
-std::vector<int> lag_pass_plan(int nsteps, int ns);
+std::vector<int> lag_pass_plan(int nsteps, int ns, int lag_ns = 0);
 void *pass_aux(plbm_lattice *h);
 void finish_run(plbm_lattice *h, int a0, int b0, int nsteps, void *latest, void *lag,
                 void *spare, int *iold, int *inew);

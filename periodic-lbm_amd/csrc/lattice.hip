@@ -382,7 +382,8 @@ int launch_step_cfg(plbm_lattice *h, int model, const void *src, void *dst, Laun
     });
 }
 
-int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew);
+int run_steps(plbm_lattice *h, int model, int ns, int lag_ns, int nsteps, int *iold,
+              int *inew);
 int default_steps_per_launch(const plbm_lattice *h);
 
 int launch_step(plbm_lattice *h, int model, const void *src, void *dst) {
@@ -503,6 +504,40 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
         if (status == PLBM_OK && tmin < tbest) {
             tbest = tmin;
             best.ns = ns;
+        }
+    }
+    // depth of a run's last pass, which also stores the lagged state: a run of n steps
+    // costs about (n - L) x tbest for its other passes plus that pass's time t(L), so the
+    // depth L >= ns that minimises t(L) - L x tbest.  Where that pass is bound by its
+    // writes (TRT FP64: about 3 ms at any depth at 8192^2) the deepest wins, and 20 steps
+    // run as 6 + 6 + 8 whether ns came out as 7 or 8.  The lag level goes to the
+    // lattice's third buffer, whose contents are not needed between runs.
+    best.lag_ns = 0;
+    void *aux = best.ns > 1 && status == PLBM_OK ? pass_aux(h) : nullptr;
+    if (aux != nullptr) {
+        double cbest = 1e300;
+        for (int L = best.ns; L <= max_steps_per_pass(h->storage) && status == PLBM_OK; ++L) {
+            double tmin = 1e300;
+            for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
+                status = launch_multi(h, model, L, h->f[src], scratch, aux);  // warm
+                if (status != PLBM_OK) break;
+                (void)hipEventRecord(e0, h->stream);
+                for (int r = 0; r < reps && status == PLBM_OK; ++r)
+                    status = launch_multi(h, model, L, h->f[src], scratch, aux);
+                (void)hipEventRecord(e1, h->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("autotune: event synchronisation failed");
+                    status = PLBM_ERR_HIP;
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                tmin = std::min(tmin, double(ms) / reps);
+            }
+            if (status == PLBM_OK && tmin - L * tbest < cbest) {
+                cbest = tmin - L * tbest;
+                best.lag_ns = L;
+            }
         }
     }
     if (status != PLBM_OK) return status;
@@ -757,7 +792,7 @@ int plbm_run(plbm_lattice *h, int model, int nsteps, int *iold, int *inew) try {
     PLBM_REQUIRE(*iold != *inew, PLBM_ERR_ARG, "iold == inew");
     PLBM_TRY(require_params(h));
     const int ns = h->cfg[model].ns > 0 ? h->cfg[model].ns : default_steps_per_launch(h);
-    return run_steps(h, model, ns, nsteps, iold, inew);
+    return run_steps(h, model, ns, h->cfg[model].lag_ns, nsteps, iold, inew);
 } catch (...) {
     return plbm::host_exception();
 }
@@ -896,7 +931,7 @@ int plbm_sync(plbm_lattice *h) try {
 int plbm_set_launch(plbm_lattice *h, int model, int block, int xcd_remap) try {
     PLBM_REQUIRE(h, PLBM_ERR_ARG, "null lattice handle");
     PLBM_REQUIRE(model >= PLBM_BGK && model <= PLBM_BGK_SPLIT, PLBM_ERR_ARG, "unknown collision model");
-    LaunchCfg c{block, xcd_remap, h->cfg[model].ns};
+    LaunchCfg c{block, xcd_remap, h->cfg[model].ns, h->cfg[model].lag_ns};
     PLBM_REQUIRE(valid_cfg(c), PLBM_ERR_ARG,
                  "block must be a multiple of 64 in [64,1024] and xcd_remap 0 or 1");
     h->cfg[model] = c;
@@ -912,6 +947,7 @@ int plbm_set_steps_per_launch(plbm_lattice *h, int model, int ns) try {
                  "steps per launch must be 0.." + std::to_string(max_steps_per_pass(h->storage)) +
                      " for this storage");
     h->cfg[model].ns = ns;
+    h->cfg[model].lag_ns = 0;  // the chosen depth holds for the last pass as well
     return PLBM_OK;
 } catch (...) {
     return plbm::host_exception();

@@ -57,7 +57,7 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nwg) {
 }
 
 template <int MODEL, int ST, int NS>
-__global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
+__global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m, int sync) {
     // wave-uniform: the column walk's indices and addresses are then scalar
     const int wg = xcd_contiguous(blockIdx.x, gridDim.x);
     const int gw = __builtin_amdgcn_readfirstlane(wg * 4 + (threadIdx.x >> 6));
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m) {
     // (8192^2 FP64 TRT, same-box A/B, ms per pass: 6 steps 2.11 -> 2.03, 7 steps 2.40 ->
     // 2.26; FP32 storage and BGK measured slower with it, so it stays with FP64 TRT)
     constexpr bool rot = MODEL == PLBM_TRT && ST == PLBM_F64 && (NS == 6 || NS == 7);
-    if (x1 > x0) multi_wave<MODEL, ST, NS, rot>(a, band, x0, x1 - x0);
+    if (x1 > x0) multi_wave<MODEL, ST, NS, rot>(a, band, x0, x1 - x0, sync != 0);
 }
 
 int max_steps_per_pass(int) { return kMaxPassSteps; }
@@ -110,8 +110,16 @@ int launch_multi(plbm_lattice *h, int model, int ns, const void *src, void *dst,
                                                    : default_pass_segments(h->nx);
                     const long long nw = (long long)waves_y * m;
                     const dim3 grid((unsigned)((nw + 3) / 4));
+                    // A barrier per column keeps a workgroup's four wavefronts (adjacent
+                    // bands) in step.  It pays in the pass that also stores the lagged level,
+                    // bound by its two store streams, and not elsewhere (8192^2 TRT FP64,
+                    // same-box A/B, ms per launch without / with: lag-writing pass of 8 steps
+                    // 2.99 / 2.92, pass of 6 steps 1.99 / 1.99, pass of 8 steps 2.62 / 2.69).
+                    // It needs every wavefront of every workgroup to walk the same number
+                    // of columns: equal segments and no idle wavefronts.
+                    const int sync = lag != nullptr && h->nx % m == 0 && nw % 4 == 0;
                     hipLaunchKernelGGL((k_multi<MM, SS, NS>), grid, dim3(256), 0, h->stream, a,
-                                       waves_y, m);
+                                       waves_y, m, sync);
                 }
                 return NS <= max_steps_per_pass<SS>();
             };
@@ -154,8 +162,9 @@ std::vector<int> pass_plan(int nsteps, int ns) {
 // second store: 3.02 / 1.97, 3.08 / 2.02, 3.03 / 2.01, 2.93 / 1.98, 2.92 / 2.06, 3.00 /
 // 2.40, 2.99 / 2.62), so it takes as many steps as a pass holds and the rest is split as
 // pass_plan does: 20 steps at depth 8 as 6 + 6 + 8 (7.1 ms) instead of 6 + 7 + 7 (7.5 ms).
-std::vector<int> lag_pass_plan(int nsteps, int ns) {
-    const int last = std::min(ns, nsteps);
+// lag_ns (> 0) sets that last pass's depth apart from ns (the autotuner's choice).
+std::vector<int> lag_pass_plan(int nsteps, int ns, int lag_ns) {
+    const int last = std::min(lag_ns > 0 ? lag_ns : ns, nsteps);
     std::vector<int> d = nsteps > last ? pass_plan(nsteps - last, ns) : std::vector<int>();
     d.push_back(last);
     return d;
@@ -214,7 +223,8 @@ static int timing_mark(plbm_lattice *h, int depth) {
     return PLBM_OK;
 }
 
-int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *inew) {
+int run_steps(plbm_lattice *h, int model, int ns, int lag_ns, int nsteps, int *iold,
+              int *inew) {
     h->run_depths.clear();
     PLBM_TRY(timing_mark(h, 0));
     if (ns <= 1 || nsteps < 2) {
@@ -242,7 +252,7 @@ int run_steps(plbm_lattice *h, int model, int ns, int nsteps, int *iold, int *in
         finish_run(h, a0, b0, nsteps, other, latest, nullptr, iold, inew);
         return PLBM_OK;
     }
-    const std::vector<int> plan = lag_pass_plan(nsteps, ns);
+    const std::vector<int> plan = lag_pass_plan(nsteps, ns, lag_ns);
     for (size_t p = 0; p + 1 < plan.size(); ++p) {
         PLBM_TRY(launch(plan[p], nullptr));
         std::swap(latest, other);

@@ -214,9 +214,13 @@ __device__ __forceinline__ R pull_y(int q, R v) {
 // [x0, x0 + ncols) (0 <= x0, x0 + ncols <= nx) of the grid described by `a` NS steps,
 // a.src -> a.dst, and, when a.lag is set, stores level NS-1 of the same cells there.
 // All 64 lanes must call it together (lane shifts and shuffles).  ROT: see the steady
-// state below (the arithmetic and results are the same either way).
+// state below (the arithmetic and results are the same either way).  sync: a workgroup
+// barrier at the start of every column, which keeps the workgroup's wavefronts (adjacent
+// bands) on the same column; the caller sets it only where every wavefront of every
+// workgroup walks the same number of columns, so that all of them reach every barrier.
 template <int MODEL, int ST, int NS, bool ROT = false>
-__device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols) {
+__device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, int ncols,
+                                           bool sync = false) {
     using T = typename Traits<ST>::T;
     using R = typename Traits<ST>::R;
     constexpr int VR = multi_vr<ST, NS>();
@@ -318,6 +322,7 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     // a level's newest column, 1 and 2 the older ones; a level that advances shifts them.
     const int shift_end = ROT ? 2 * NS : iters;
     for (int i = 0; i < shift_end; ++i) {
+        if (sync) __builtin_amdgcn_s_barrier();
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             W[0][2][q] = W[0][1][q];
@@ -351,6 +356,7 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
         auto steady = [&](int i, auto phase) __attribute__((always_inline)) {
             constexpr int P = decltype(phase)::value;
             constexpr int s0 = (3 - P - 1) % 3, s1 = (4 - P - 1) % 3, s2 = (5 - P - 1) % 3;
+            if (sync) __builtin_amdgcn_s_barrier();
             load_level0(i, (2 * NS + P) % 3, s0, (2 * NS + P + ahead) % 3);
 #pragma unroll
             for (int k = 1; k <= NS; ++k) {

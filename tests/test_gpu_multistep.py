@@ -199,12 +199,33 @@ def test_automatic_work_split(P, gpu, nx, ny):
 
 
 def test_autotune_picks_a_depth(P, gpu):
+    """The autotuner picks a pass depth and, separately, the depth of the pass that ends
+    a run (it stores the lagged state); a run under that choice equals single steps bit
+    for bit, and setting a depth afterwards applies it to the last pass too."""
     n = 2048
     lat = P.Lattice(n, n, 2, "f64")
     lat.set_params(1.9, 0.25)
     lat.equilibrium_init(0, *fields(n, n, 2))
     ms = lat.autotune("trt", 0, reps=3)
-    assert ms > 0 and lat.get_steps_per_launch("trt") in range(1, 9)
+    ns = lat.get_steps_per_launch("trt")
+    assert ms > 0 and ns in range(1, 9)
+    ref = P.Lattice(n, n, 2, "f64")
+    ref.set_params(1.9, 0.25)
+    ref.equilibrium_init(0, *fields(n, n, 2))
+    ref.set_steps_per_launch("trt", 1)
+    ri, rn = ref.run("trt", 20, 0, 1)
+    lat.set_run_timing(True)
+    i, n_ = lat.run("trt", 20, 0, 1)
+    depths = [d for d, _ in lat.run_launch_times()]
+    print("autotuned depth", ns, "run of 20 steps", depths, flush=True)
+    assert sum(depths) == 20 and (i, n_) == (ri, rn)
+    if ns > 1:
+        assert depths[-1] >= ns and all(d <= ns for d in depths[:-1])
+    np.testing.assert_array_equal(lat.download_f(i), ref.download_f(ri))
+    np.testing.assert_array_equal(lat.download_f(n_), ref.download_f(rn))
+    lat.set_steps_per_launch("trt", 4)
+    lat.run("trt", 20, 0, 1)
+    assert [d for d, _ in lat.run_launch_times()] == [4] * 5
     lat.set_steps_per_launch("trt", 8)
     with pytest.raises(P.PlbmError):
         lat.set_steps_per_launch("trt", 9)  # passes hold at most 8 steps
