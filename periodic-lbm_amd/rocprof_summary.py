@@ -47,10 +47,13 @@ def stats(args):
 
 def trace(args):
     """Per-kernel durations from kernel_trace.csv, restricted to the last --last dispatches
-    of each selected kernel (the timed region of bench.py, after warm-up and tuning)."""
+    of each selected kernel (the timed region of bench.py, after warm-up and tuning), or
+    with --first to the first ones (the autotuner's timing of each pass depth, which
+    precedes its timing of the passes that also store the lagged state)."""
     rows = _rows(args.csv)
+    which = f"First {args.first}" if args.first else f"Last {args.last}"
     lines = [f"# rocprofv3 kernel trace: dispatch durations", "", f"Command: `{args.cmd}`", "",
-             f"Last {args.last} dispatches of each kernel below"
+             f"{which} dispatches of each kernel below"
              + (f" with grid size {', '.join(map(str, args.grid_size))}" if args.grid_size else "")
              + ".", "",
              "| kernel | dispatches | avg us | min us | max us | bytes/launch | GB/s at avg |",
@@ -62,7 +65,7 @@ def trace(args):
              and (not args.grid_size or int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
                   in args.grid_size)]
         d.sort()
-        sel = [x[1] for x in d[-args.last:]]
+        sel = [x[1] for x in (d[:args.first] if args.first else d[-args.last:])]
         if not sel:
             raise SystemExit(f"no dispatch of {sub!r}")
         avg = sum(sel) / len(sel)
@@ -269,6 +272,8 @@ def main():
     c.add_argument("--kernel", action="append", required=True,
                    help="<name substring>=<algorithmic bytes per launch>, repeatable")
     c.add_argument("--last", type=int, default=50)
+    c.add_argument("--first", type=int, default=0,
+                   help="the first N dispatches of each kernel instead of the last")
     c.add_argument("--grid-size", type=int, action="append", default=[],
                    help="only dispatches with one of these grid sizes (threads)")
     c.add_argument("--cmd", required=True)

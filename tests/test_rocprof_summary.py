@@ -42,6 +42,18 @@ def test_trace_mode_averages_the_last_dispatches_of_one_grid(tmp_path):
     assert cells[-2] == "2.0"  # 1000 B / 500 ns = 2.0 GB/s
 
 
+def test_trace_mode_first_dispatches(tmp_path):
+    spec = [("k_multi<1, 0, 8>", 1000 * i, 400 + 100 * i, 7) for i in range(6)]
+    write_csv(tmp_path / "t.csv", trace_rows(spec))
+    run("trace", tmp_path / "t.csv", tmp_path / "t.md", "--kernel", "k_multi<1, 0, 8>=1000",
+        "--first", 2, "--cmd", "c")
+    text = (tmp_path / "t.md").read_text()
+    assert "First 2 dispatches" in text
+    row = [l for l in text.splitlines() if "k_multi" in l][0]
+    cells = [c.strip() for c in row.split("|")]
+    assert cells[2:6] == ["2", "0.5", "0.4", "0.5"]  # dispatches, avg, min, max us
+
+
 def test_window_mode_sums_the_timed_dispatches(tmp_path):
     spec = [("other", 0, 10, 1), ("k_multi<1, 0, 5>", 100, 4000, 1),
             ("k_multi<1, 0, 6>", 5000, 2000, 1), ("k_multi<1, 0, 6>", 8000, 2200, 1),
