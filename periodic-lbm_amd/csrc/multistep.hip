@@ -64,10 +64,13 @@ __global__ void __launch_bounds__(256) k_multi(StepArgs a, int waves_y, int m, i
     if (gw >= waves_y * m) return;  // the whole wavefront
     const int band = gw % waves_y, j = gw / waves_y;
     const int x0 = (int)((long long)a.nx * j / m), x1 = (int)((long long)a.nx * (j + 1) / m);
-    // the unrolled steady state where it fits two wavefronts per SIMD and measured faster
-    // (8192^2 FP64 TRT, same-box A/B, ms per pass: 6 steps 2.11 -> 2.03, 7 steps 2.40 ->
-    // 2.26; FP32 storage and BGK measured slower with it, so it stays with FP64 TRT)
-    constexpr bool rot = MODEL == PLBM_TRT && ST == PLBM_F64 && (NS == 6 || NS == 7);
+    // the unrolled steady state where it keeps the wavefronts per SIMD and measured faster
+    // (8192^2 TRT, same-box A/B, ms per pass: FP64 6 steps 2.11 -> 2.03, 7 steps 2.40 ->
+    // 2.26; FP32 storage 8 steps 1.45 -> 1.43, FP16 1.40 -> 1.37, their 20-step runs
+    // 1.5% faster; FP32 storage at 6 steps and BGK measured slower with it, FP64 at 8
+    // steps needs more than 256 registers)
+    constexpr bool rot = MODEL == PLBM_TRT && ((ST == PLBM_F64 && (NS == 6 || NS == 7)) ||
+                                              ((ST == PLBM_F32S || ST == PLBM_F16S) && NS == 8));
     if (x1 > x0) multi_wave<MODEL, ST, NS, rot>(a, band, x0, x1 - x0, sync != 0);
 }
 

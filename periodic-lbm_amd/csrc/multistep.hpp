@@ -350,8 +350,9 @@ __device__ __forceinline__ void multi_wave(const StepArgs &a, int wy, int x0, in
     // P = (i - 2 NS) % 3; the start-up leaves (newest, -1, -2) in slots (0, 1, 2), so in
     // phase P column -back sits in slot (back - P - 1) mod 3 of every level.  The walk is
     // rounded up to whole groups of three.  It costs registers (FP64 TRT: 206 -> 227 at 6
-    // steps, 230 -> 255 at 7, beyond 256 at 8), so the pass kernel asks for it only where
-    // it still fits two wavefronts per SIMD and measured faster.
+    // steps, 230 -> 255 at 7, beyond 256 at 8; FP32 / FP16 storage at 8 steps: 153 -> 163,
+    // 151 -> 160), so the pass kernel asks for it only where it keeps the wavefronts per
+    // SIMD and measured faster.
     if constexpr (ROT) {
         auto steady = [&](int i, auto phase) __attribute__((always_inline)) {
             constexpr int P = decltype(phase)::value;
