@@ -2,17 +2,22 @@
 
   python periodic-lbm_amd/rocprof_summary.py stats  <kernel_stats.csv> <out.md> --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py trace  <kernel_trace.csv> <out.md>
-         --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N --cmd "<cmd>"
+         --kernel "<substring>=<bytes per launch>" [--kernel ...] --last N | --first N
+         --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py window <kernel_trace.csv> <out.md>
          --sequence <name substring> ... [--bytes B ...] --skip K --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py span <kernel_trace.csv> <out.md>
          --kernel <substring> [--kernel ...] [--cells N --steps K] --cmd "<cmd>"
   python periodic-lbm_amd/rocprof_summary.py sq <counter_collection.csv> <kernel_trace.csv>
-         <out.md> --kernel <substring> [--kernel ...] --grid-size N --cmd "<pmc cmd>"
+         <out.md> --kernel <substring>[@first N] [--kernel ...] --grid-size N --cmd "<pmc cmd>"
          --trace-cmd "<trace cmd>"
   python periodic-lbm_amd/rocprof_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> <out.json> --kernel <substring> --cells N
-         --bytes-per-lu B --cmd "<cmd>"
+         --bytes-per-lu B [--first N] --cmd "<cmd>"
+
+The pass kernels' first dispatches in bench.py are the autotuner's timing of plain passes
+of each depth; later ones of the same kernel can be passes that also store the lagged
+state (the autotuner's timing of the last pass of a run, the timed run's last pass).
 
 The traffic record calibrates FETCH_SIZE / WRITE_SIZE (KiB per dispatch) against the
 engine's copy probe (k_copy, which reads and writes a known number of bytes in the same
@@ -111,19 +116,22 @@ def _per_kernel(rows, counter, grid_size=None):
         if r["Counter_Name"] != counter:
             continue
         agg.setdefault(r["Kernel_Name"], []).append(
-            (float(r["Counter_Value"]), int(r.get("Grid_Size") or 0)))
+            (float(r["Counter_Value"]), int(r.get("Grid_Size") or 0),
+             int(r.get("Dispatch_Id") or 0)))
     return agg
 
 
-def _values(agg, name, grid_size=None):
-    return [v for v, g in agg[name] if grid_size is None or g == grid_size]
+def _values(agg, name, grid_size=None, first=0):
+    """The kernel's values in dispatch order; with first, only its first dispatches."""
+    sel = sorted((d, v) for v, g, d in agg[name] if grid_size is None or g == grid_size)
+    return [v for _, v in (sel[:first] if first else sel)]
 
 
-def _pick(agg, sub, grid_size=None):
+def _pick(agg, sub, grid_size=None, first=0):
     keys = [k for k in agg if sub in k]
     if not keys:
         raise SystemExit(f"no kernel matching {sub!r} in {list(agg)}")
-    vals = _values(agg, keys[0], grid_size)
+    vals = _values(agg, keys[0], grid_size, first)
     if not vals:
         raise SystemExit(f"no dispatch of {keys[0]!r} with grid size {grid_size}")
     return keys[0], statistics.median(vals)
@@ -137,14 +145,14 @@ def traffic(args):
     copy_bytes = args.copy_bytes
     f_scale = copy_bytes / (copy_fetch * 1024.0)
     w_scale = copy_bytes / (copy_write * 1024.0)
-    kname, kfetch = _pick(fetch, args.kernel, args.grid_size)
-    _, kwrite = _pick(write, args.kernel, args.grid_size)
+    kname, kfetch = _pick(fetch, args.kernel, args.grid_size, args.first)
+    _, kwrite = _pick(write, args.kernel, args.grid_size, args.first)
     read_b = kfetch * 1024.0 * f_scale
     write_b = kwrite * 1024.0 * w_scale
     algo = args.cells * args.bytes_per_lu
     rec = {
         "kernel": kname, "command": args.cmd, "grid_size": args.grid_size,
-        "dispatches": len(_values(fetch, kname, args.grid_size)),
+        "dispatches": len(_values(fetch, kname, args.grid_size, args.first)),
         "fetch_kib_per_dispatch": kfetch, "write_kib_per_dispatch": kwrite,
         "calibration": {"kernel": copy_name, "bytes_read_and_written": copy_bytes,
                         "fetch_kib": copy_fetch, "write_kib": copy_write,
@@ -225,18 +233,28 @@ def sq(args):
              f"SQ_INSTS_VALU x 4 / ({args.simds} SIMDs x {args.clock_ghz} GHz x launch time).",
              "", "| kernel | VALU | SALU | LDS | VMEM rd | VMEM wr | waves | launch ms | VALU busy |",
              "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
-    for sub in args.kernel:
+    for spec in args.kernel:
+        # "<substring>@N": only the kernel's first N dispatches (in each file)
+        sub, _, first = spec.rpartition("@") if "@" in spec else (spec, "", "")
+        first = int(first) if first else 0
+        rows = [r for r in crow if sub in r["Kernel_Name"] and r["Counter_Name"] in SQ_COUNTERS
+                and grid_ok(r, "Grid_Size")]
+        if first:
+            ids = sorted({int(r["Dispatch_Id"]) for r in rows})[:first]
+            rows = [r for r in rows if int(r["Dispatch_Id"]) in set(ids)]
         vals = {}
         name = None
-        for r in crow:
-            if sub in r["Kernel_Name"] and r["Counter_Name"] in SQ_COUNTERS and grid_ok(r, "Grid_Size"):
-                vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-                name = r["Kernel_Name"]
+        for r in rows:
+            vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            name = r["Kernel_Name"]
         if name is None:
             raise SystemExit(f"no counter rows for {sub!r}")
         med = {k: statistics.median(v) for k, v in vals.items()}
-        durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trows
-                if sub in r["Kernel_Name"] and grid_ok(r, "Grid_Size_X")]
+        tsel = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in trows
+                      if sub in r["Kernel_Name"] and grid_ok(r, "Grid_Size_X"))
+        if first:
+            tsel = tsel[:first]
+        durs = [e - b for b, e in tsel]
         if not durs:
             raise SystemExit(f"no trace dispatch of {sub!r}")
         t = statistics.median(durs) * 1e-9
@@ -265,6 +283,8 @@ def main():
     b.add_argument("--copy-bytes", type=float, default=float(4 << 30))
     b.add_argument("--grid-size", type=int, default=None,
                    help="only dispatches of the kernel with this grid size (threads)")
+    b.add_argument("--first", type=int, default=0,
+                   help="only the kernel's first N dispatches (the copy probe: all)")
     b.add_argument("--cmd", required=True)
     c = sub.add_parser("trace")
     c.add_argument("csv")
@@ -300,7 +320,8 @@ def main():
     q.add_argument("counters", help="counter_collection.csv of a --pmc run of SQ_ counters")
     q.add_argument("trace", help="kernel_trace.csv of a run of the same command")
     q.add_argument("out")
-    q.add_argument("--kernel", action="append", required=True, help="name substring, repeatable")
+    q.add_argument("--kernel", action="append", required=True,
+                   help="name substring, repeatable; <substring>@N for the first N dispatches")
     q.add_argument("--grid-size", type=int, action="append", default=[],
                    help="only dispatches with one of these grid sizes (threads)")
     q.add_argument("--simds", type=int, default=1024)

@@ -113,3 +113,39 @@ def test_sq_mode_reports_valu_busy(tmp_path):
         "k_multi<1, 0, 8>", "--cmd", "c", "--trace-cmd", "t")
     row = [l for l in (tmp_path / "sq.md").read_text().splitlines() if "k_multi" in l][0]
     assert row.rstrip(" |").endswith("0.50")  # 1 ms of VALU issue in a 2 ms launch
+
+
+def test_sq_mode_first_dispatches(tmp_path):
+    # dispatch 1: 1 ms of VALU issue; dispatch 2 (a later, different pass): 3 ms
+    unit = 1024 * 2.4e9 * 1e-3 / 4
+    counters = [{"Kernel_Name": "k_multi<1, 0, 8>", "Counter_Name": c, "Counter_Value": v * k,
+                 "Grid_Size": 7, "Dispatch_Id": k}
+                for k in (1, 3)
+                for c, v in (("SQ_INSTS_VALU", unit), ("SQ_INSTS_SALU", 1e6), ("SQ_WAVES", 64))]
+    write_csv(tmp_path / "c.csv", counters)
+    write_csv(tmp_path / "t.csv", trace_rows([("k_multi<1, 0, 8>", 0, 2000000, 7),
+                                              ("k_multi<1, 0, 8>", 5000000, 4000000, 7)]))
+    run("sq", tmp_path / "c.csv", tmp_path / "t.csv", tmp_path / "sq.md", "--kernel",
+        "k_multi<1, 0, 8>@1", "--cmd", "c", "--trace-cmd", "t")
+    row = [l for l in (tmp_path / "sq.md").read_text().splitlines() if "k_multi" in l][0]
+    cells = [c.strip() for c in row.split("|")]
+    assert cells[2] == f"{unit / 1e6:.2f}" and cells[-3] == "2.0000"  # VALU, launch ms
+    assert row.rstrip(" |").endswith("0.50")
+
+
+def test_traffic_mode_first_dispatches(tmp_path):
+    def rows(counter, copy_kib, kern_kibs):
+        out = [{"Kernel_Name": "plbm::k_copy", "Counter_Name": counter, "Counter_Value": copy_kib,
+                "Grid_Size": 1, "Dispatch_Id": 1}]
+        out += [{"Kernel_Name": "plbm::k_multi<1, 0, 8>", "Counter_Name": counter,
+                 "Counter_Value": v, "Grid_Size": 700416, "Dispatch_Id": 2 + i}
+                for i, v in enumerate(kern_kibs)]
+        return out
+    # two plain passes first, then three that also store the lagged state (twice the writes)
+    write_csv(tmp_path / "f.csv", rows("FETCH_SIZE", 2048, [1000, 1000, 1000, 1000, 1000]))
+    write_csv(tmp_path / "w.csv", rows("WRITE_SIZE", 4096, [3000, 3000, 6000, 6000, 6000]))
+    run("traffic", tmp_path / "f.csv", tmp_path / "w.csv", tmp_path / "r.json", "--kernel",
+        "k_multi<1, 0, 8>", "--cells", 100, "--bytes-per-lu", 144, "--copy-bytes", 4 << 20,
+        "--grid-size", 700416, "--first", 2, "--cmd", "c")
+    rec = json.loads((tmp_path / "r.json").read_text())
+    assert rec["dispatches"] == 2 and rec["write_bytes_per_launch"] == 3000 * 1024
