@@ -162,8 +162,9 @@ std::vector<int> pass_plan(int nsteps, int ns) {
 // Depths of a run of nsteps >= 2 steps whose last pass also stores the state one step
 // before the latest: that pass writes the whole state twice and is bound by HBM writes
 // whatever its depth (8192^2 TRT FP64, ms per launch for d = 2..8 with / without the
-// second store: 3.02 / 1.97, 3.08 / 2.02, 3.03 / 2.01, 2.93 / 1.98, 2.92 / 2.06, 3.00 /
-// 2.40, 2.99 / 2.62), so it takes as many steps as a pass holds and the rest is split as
+// second store, before that pass had its per-column barrier: 3.02 / 1.97, 3.08 / 2.02,
+// 3.03 / 2.01, 2.93 / 1.98, 2.92 / 2.06, 3.00 / 2.40, 2.99 / 2.62), so it takes as many
+// steps as a pass holds and the rest is split as
 // pass_plan does: 20 steps at depth 8 as 6 + 6 + 8 (7.1 ms) instead of 6 + 7 + 7 (7.5 ms).
 // lag_ns (> 0) sets that last pass's depth apart from ns (the autotuner's choice).
 std::vector<int> lag_pass_plan(int nsteps, int ns, int lag_ns) {
