@@ -449,60 +449,49 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
     }
     PLBM_HIP(hipEventCreate(&e0));
     PLBM_HIP(hipEventCreate(&e1));
+    // ms per launch of launch(): the best of three trials of `reps` launches, each trial
+    // after one warm-up launch, timed with events on the lattice's stream
+    auto time_launch = [&](auto &&launch, double *ms_out) -> int {
+        double tmin = 1e300;
+        for (int trial = 0; trial < 3; ++trial) {
+            PLBM_TRY(launch());  // warm
+            (void)hipEventRecord(e0, h->stream);
+            for (int r = 0; r < reps; ++r) PLBM_TRY(launch());
+            (void)hipEventRecord(e1, h->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("autotune: event synchronisation failed");
+                return PLBM_ERR_HIP;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tmin = std::min(tmin, double(ms) / reps);
+        }
+        *ms_out = tmin;
+        return PLBM_OK;
+    };
     LaunchCfg best;
     double tbest = 1e300;
-    int status = PLBM_OK;
     for (int block : {64, 128, 256, 512, 1024}) {
         for (int remap : {0, 1}) {
-            LaunchCfg c{block, remap};
-            double tmin = 1e300;
-            for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
-                status = launch_step_cfg(h, model, h->f[src], scratch, c);  // warm
-                if (status != PLBM_OK) break;
-                (void)hipEventRecord(e0, h->stream);
-                for (int r = 0; r < reps && status == PLBM_OK; ++r)
-                    status = launch_step_cfg(h, model, h->f[src], scratch, c);
-                (void)hipEventRecord(e1, h->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) {
-                    set_error("autotune: event synchronisation failed");
-                    status = PLBM_ERR_HIP;
-                    break;
-                }
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                tmin = std::min(tmin, double(ms) / reps);
-            }
-            if (status != PLBM_OK) break;
-            if (tmin < tbest) {
-                tbest = tmin;
+            const LaunchCfg c{block, remap};
+            double t;
+            PLBM_TRY(time_launch(
+                [&] { return launch_step_cfg(h, model, h->f[src], scratch, c); }, &t));
+            if (t < tbest) {
+                tbest = t;
                 best = c;
             }
         }
-        if (status != PLBM_OK) break;
     }
     // steps per pass of plbm_run: the single-step kernel at the shape just chosen against
     // passes of 2..8 steps, compared per lattice update
     best.ns = 1;
-    for (int ns = 2; ns <= max_steps_per_pass(h->storage) && status == PLBM_OK; ++ns) {
-        double tmin = 1e300;
-        for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
-            status = launch_multi(h, model, ns, h->f[src], scratch);  // warm
-            if (status != PLBM_OK) break;
-            (void)hipEventRecord(e0, h->stream);
-            for (int r = 0; r < reps && status == PLBM_OK; ++r)
-                status = launch_multi(h, model, ns, h->f[src], scratch);
-            (void)hipEventRecord(e1, h->stream);
-            if (hipEventSynchronize(e1) != hipSuccess) {
-                set_error("autotune: event synchronisation failed");
-                status = PLBM_ERR_HIP;
-                break;
-            }
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            tmin = std::min(tmin, double(ms) / reps / ns);
-        }
-        if (status == PLBM_OK && tmin < tbest) {
-            tbest = tmin;
+    for (int ns = 2; ns <= max_steps_per_pass(h->storage); ++ns) {
+        double t;
+        PLBM_TRY(time_launch([&] { return launch_multi(h, model, ns, h->f[src], scratch); }, &t));
+        if (t / ns < tbest) {
+            tbest = t / ns;
             best.ns = ns;
         }
     }
@@ -513,34 +502,19 @@ int autotune_step(plbm_lattice *h, int model, int src, int reps, LaunchCfg *best
     // run as 6 + 6 + 8 whether ns came out as 7 or 8.  The lag level goes to the
     // lattice's third buffer, whose contents are not needed between runs.
     best.lag_ns = 0;
-    void *aux = best.ns > 1 && status == PLBM_OK ? pass_aux(h) : nullptr;
+    void *aux = best.ns > 1 ? pass_aux(h) : nullptr;
     if (aux != nullptr) {
         double cbest = 1e300;
-        for (int L = best.ns; L <= max_steps_per_pass(h->storage) && status == PLBM_OK; ++L) {
-            double tmin = 1e300;
-            for (int trial = 0; trial < 3 && status == PLBM_OK; ++trial) {
-                status = launch_multi(h, model, L, h->f[src], scratch, aux);  // warm
-                if (status != PLBM_OK) break;
-                (void)hipEventRecord(e0, h->stream);
-                for (int r = 0; r < reps && status == PLBM_OK; ++r)
-                    status = launch_multi(h, model, L, h->f[src], scratch, aux);
-                (void)hipEventRecord(e1, h->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) {
-                    set_error("autotune: event synchronisation failed");
-                    status = PLBM_ERR_HIP;
-                    break;
-                }
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                tmin = std::min(tmin, double(ms) / reps);
-            }
-            if (status == PLBM_OK && tmin - L * tbest < cbest) {
-                cbest = tmin - L * tbest;
+        for (int L = best.ns; L <= max_steps_per_pass(h->storage); ++L) {
+            double t;
+            PLBM_TRY(time_launch(
+                [&] { return launch_multi(h, model, L, h->f[src], scratch, aux); }, &t));
+            if (t - L * tbest < cbest) {
+                cbest = t - L * tbest;
                 best.lag_ns = L;
             }
         }
     }
-    if (status != PLBM_OK) return status;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = {best, tbest};
